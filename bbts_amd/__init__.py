@@ -1,0 +1,159 @@
+"""bbts_amd -- MI355X block kernels of BarbaTOS behind a C ABI.
+
+The compute path is libbbts_hip.so (hand-written HIP for gfx950, built in-tree
+by ``python -m bbts_amd.build``); the host runtime mirror of the reference's
+ud_impl_t / udf_manager / storage / reservation station is libbbts_runtime.so
+(C++).  This module is a thin ctypes binding of both for tests, the bench and
+Python callers.  There is no fallback: if the library is missing or no GPU is
+visible, the calls raise.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbbts_hip.so")
+RT_PATH = os.path.join(_HERE, "libbbts_runtime.so")
+
+META_BYTES = 260
+
+# precision modes (include/bbts_hip.h)
+PREC_FP32, PREC_BF16X1, PREC_BF16X3, PREC_BF16X6, PREC_AUTO, PREC_FP8X1, PREC_FP8X3 = range(7)
+# schedule policies
+SCHED_FIFO, SCHED_HEURISTIC, SCHED_TUNED, SCHED_LEARNED = range(4)
+SCHED_FORCED_CFG0 = 100
+
+STATUS = {0: "ok", 1: "invalid argument", 2: "shape mismatch", 3: "unknown block", 4: "out of device memory",
+          5: "HIP runtime error", 6: "unsupported", 7: "wrong tensor format"}
+
+
+class BBTSError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("bbts_hip error %d (%s): %s" % (code, STATUS.get(code, "?"), msg))
+        self.code = code
+
+
+_lib = None
+_rt = None
+
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+_sz = ctypes.c_size_t
+_f = ctypes.c_float
+_d = ctypes.c_double
+_pvp = ctypes.POINTER(ctypes.c_void_p)
+
+# name -> argtypes (all return int unless listed in _RESTYPES)
+_SIGS = {
+    "bbts_hip_init": [ctypes.c_int, _pvp],
+    "bbts_hip_shutdown": [_vp],
+    "bbts_hip_strerror": [ctypes.c_int],
+    "bbts_hip_last_error": [],
+    "bbts_hip_sync": [_vp, _vp],
+    "bbts_hip_device_info": [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_sz)],
+    "bbts_hip_dense_size": [_u32, _u32],
+    "bbts_hip_block_alloc": [_vp, _vp, _sz, _pvp],
+    "bbts_hip_block_free": [_vp, _vp],
+    "bbts_hip_block_register": [_vp, _vp, _sz, _vp, _pvp],
+    "bbts_hip_block_meta": [_vp, _vp, _vp, ctypes.POINTER(_sz)],
+    "bbts_hip_block_set_meta": [_vp, _vp, _vp, _vp],
+    "bbts_hip_block_data": [_vp],
+    "bbts_hip_block_upload": [_vp, _vp, _vp, _sz, _vp],
+    "bbts_hip_block_download": [_vp, _vp, _vp, _sz, _vp],
+    "bbts_hip_pool_stats": [_vp, ctypes.POINTER(_sz), ctypes.POINTER(_sz)],
+    "bbts_hip_dense_matmul": [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp],
+    "bbts_hip_dense_add": [_vp, _vp, _vp, _vp, _vp],
+    "bbts_hip_dense_reduce": [_vp, _pvp, ctypes.c_int, _vp, _vp],
+    "bbts_hip_broadcast": [_vp, _vp, _pvp, ctypes.c_int, _vp],
+    "bbts_hip_dense_relu": [_vp, _vp, _vp, _vp],
+    "bbts_hip_dense_uniform": [_vp, _vp, _u32, _u32, _f, _f, _u64, _vp],
+    "bbts_hip_contraction": [_vp, _vp, _vp, _vp, ctypes.POINTER(_i32), ctypes.c_int, ctypes.c_int, _vp],
+    "bbts_hip_contraction_raw": [_vp, ctypes.POINTER(_i32), ctypes.c_int, ctypes.c_int, ctypes.POINTER(_i64), _vp,
+                                 ctypes.c_int, ctypes.POINTER(_i64), _vp, _vp, ctypes.c_int, _vp],
+    "bbts_hip_gemm": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _i64, ctypes.c_int, _vp, _i64,
+                      ctypes.c_int, _vp, _i64, _f, ctypes.c_int, _vp],
+    "bbts_hip_gemm_batched": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _i64, _i64, ctypes.c_int, _vp,
+                              _i64, _i64, ctypes.c_int, _vp, _i64, _i64, ctypes.c_int, _f, ctypes.c_int, _vp],
+    "bbts_hip_add_f32": [_vp, _vp, _vp, _vp, _sz, _vp],
+    "bbts_hip_reduce_f32": [_vp, _pvp, ctypes.c_int, _vp, _sz, _vp],
+    "bbts_hip_relu_f32": [_vp, _vp, _vp, _sz, _vp],
+    "bbts_hip_copy_n": [_vp, _vp, _sz, _pvp, ctypes.c_int, _vp],
+    "bbts_hip_gen_uniform": [_vp, _vp, _sz, _u64, _u64, _f, _f, _vp],
+    "bbts_hip_gen_normal": [_vp, _vp, _sz, _u64, _u64, _d, _d, _vp],
+    "bbts_hip_f32_to_bf16": [_vp, _vp, _vp, _sz, _vp],
+    "bbts_hip_residual_f64": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _i64, ctypes.c_int, _vp, _i64,
+                              ctypes.c_int, _vp, _i64, _vp, _vp],
+    "bbts_hip_batch_create": [_vp, _pvp],
+    "bbts_hip_batch_destroy": [_vp],
+    "bbts_hip_batch_clear": [_vp],
+    "bbts_hip_batch_add_gemm": [_vp, _vp, ctypes.c_int],
+    "bbts_hip_batch_plan": [_vp, ctypes.c_int, _vp],
+    "bbts_hip_batch_launch": [_vp, _vp],
+    "bbts_hip_batch_capture": [_vp, _vp],
+    "bbts_hip_batch_replay": [_vp, _vp],
+    "bbts_hip_num_tile_configs": [],
+    "bbts_hip_tile_config_shape": [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_int)],
+    "bbts_hip_batch_op_config": [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+    "bbts_hip_batch_num_groups": [_vp],
+    "bbts_hip_batch_group_info": [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                  ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_d)],
+    "bbts_hip_batch_launch_group": [_vp, ctypes.c_int, _vp],
+    "bbts_hip_gemm_desc_launch": [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp],
+    "bbts_hip_tune_batch": [_vp, ctypes.c_int, _vp],
+    "bbts_hip_tune_lookup": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_f)],
+    "bbts_hip_costmodel_create": [_vp, ctypes.c_int, _u64, _pvp],
+    "bbts_hip_costmodel_destroy": [_vp],
+    "bbts_hip_costmodel_num_features": [],
+    "bbts_hip_costmodel_features": [ctypes.c_int] * 6 + [_vp],
+    "bbts_hip_costmodel_train": [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _f, ctypes.c_int, ctypes.POINTER(_f), _vp],
+    "bbts_hip_costmodel_predict": [_vp, _vp, ctypes.c_int, _vp, _vp],
+    "bbts_hip_costmodel_grad": [_vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp],
+    "bbts_hip_costmodel_num_params": [_vp],
+    "bbts_hip_costmodel_get_params": [_vp, _vp, ctypes.c_int],
+    "bbts_hip_costmodel_set_params": [_vp, _vp, ctypes.c_int],
+}
+_RESTYPES = {
+    "bbts_hip_strerror": ctypes.c_char_p,
+    "bbts_hip_last_error": ctypes.c_char_p,
+    "bbts_hip_dense_size": _sz,
+    "bbts_hip_block_data": _vp,
+}
+
+
+def declared_symbols():
+    """Every function include/bbts_hip.h declares (checked against the built library by the tests)."""
+    return sorted(_SIGS)
+
+
+def lib():
+    """Load libbbts_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libbbts_hip.so is not built (run: python -m bbts_amd.build)")
+        L = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+        for name, args in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPES.get(name, ctypes.c_int)
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise BBTSError(rc, lib().bbts_hip_last_error().decode(errors="replace"))
+    return rc
+
+
+def call(name, *args):
+    return check(getattr(lib(), name)(*args))
+
+
+def build_library(jobs=None):
+    from .build import build as _build
+    return _build(jobs)

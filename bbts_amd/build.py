@@ -1,0 +1,89 @@
+"""Build libbbts_hip.so (HIP kernels + C ABI) and libbbts_runtime.so (the C++
+host runtime) in-tree with hipcc for gfx950.
+
+Usage: python -m bbts_amd.build [-j N]
+"""
+import argparse
+import concurrent.futures as cf
+import glob
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+RTSRC = os.path.join(HERE, "runtime")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libbbts_hip.so")
+RTLIB = os.path.join(HERE, "libbbts_runtime.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("BBTS_ARCH", "gfx950")
+
+# -packed-fp32-ops: keep the compiler from forming v_pk_{add,mul,fma}_f32
+HIP_FLAGS = [
+    "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-ffp-contract=fast", "-Wno-unused-command-line-argument",
+    "-I" + os.path.join(ROOT, "include"),
+]
+
+
+def _run(cmd):
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in p.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if p.returncode != 0:
+        raise RuntimeError("command failed: %s\n%s" % (" ".join(cmd), out))
+    return out
+
+
+def _compile(src, extra=()):
+    obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+    deps = [src] + glob.glob(os.path.join(os.path.dirname(src), "*.h")) + \
+        glob.glob(os.path.join(os.path.dirname(src), "*.cuh")) + [os.path.join(ROOT, "include", "bbts_hip.h")]
+    _run([HIPCC] + HIP_FLAGS + list(extra) + ["-c", src, "-o", obj])
+    return obj
+
+
+def build(jobs=None, verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(_compile, srcs))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    rt_srcs = sorted(glob.glob(os.path.join(RTSRC, "*.cc")))
+    if rt_srcs:
+        rt_flags = ["-O2", "-fPIC", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + RTSRC,
+                    "-D__HIP_PLATFORM_AMD__"]
+        with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+            rt_objs = list(ex.map(lambda s: _rt_compile(s, rt_flags), rt_srcs))
+        tmp = RTLIB + ".tmp"
+        _run(["g++", "-shared", "-fPIC", "-o", tmp] + rt_objs + ["-L" + HERE, "-lbbts_hip",
+                                                                "-Wl,-rpath,$ORIGIN", "-lpthread"])
+        os.replace(tmp, RTLIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+def _rt_compile(src, flags):
+    obj = os.path.join(BUILD, "rt_" + os.path.basename(src) + ".o")
+    _run(["g++"] + flags + ["-c", src, "-o", obj])
+    return obj
+
+
+def clean():
+    shutil.rmtree(BUILD, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-j", type=int, default=None)
+    a = ap.parse_args()
+    build(a.j, verbose=True)
+    sys.exit(0)

@@ -1,0 +1,193 @@
+// Internal declarations shared by the libbbts_hip.so translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/bbts_hip.h"
+
+namespace bbts_hip {
+
+// ---------------------------------------------------------------------------
+// Block header layout, bit-exact with the reference:
+//   struct tensor_meta_t { int32_t fmt_id; char _blob[256]; }   src/tensor/tensor.h:20-34
+//   dense meta {uint32 num_rows, num_cols} at _blob[0]         src/tensor/builtin_formats.h:10-13
+//   cutensor meta {uint32 rank; int64 dims[4]} at _blob[0]     libraries/cutensor/cu.h:60-63
+// The payload follows the header (tensor.h:50-51).
+// ---------------------------------------------------------------------------
+struct tensor_meta_t {
+  int32_t fmt_id;
+  char _blob[256];
+};
+static_assert(sizeof(tensor_meta_t) == BBTS_META_BYTES, "header must be 260 bytes");
+static_assert(offsetof(tensor_meta_t, _blob) == 4, "blob at +4");
+
+struct dense_m_t {
+  uint32_t num_rows;
+  uint32_t num_cols;
+};
+
+struct cu_shape_t {
+  uint32_t rank;
+  int64_t dims[4];
+};
+static_assert(sizeof(cu_shape_t) == 40, "cu_shape_t is 40 bytes");
+static_assert(offsetof(cu_shape_t, dims) == 8, "dims at +8 of the blob");
+
+inline dense_m_t dense_meta(const tensor_meta_t &m) {
+  dense_m_t d;
+  std::memcpy(&d, m._blob, sizeof(d));
+  return d;
+}
+inline void set_dense_meta(tensor_meta_t &m, uint32_t rows, uint32_t cols) {
+  dense_m_t d{rows, cols};
+  std::memcpy(m._blob, &d, sizeof(d));
+}
+inline cu_shape_t cu_meta(const tensor_meta_t &m) {
+  cu_shape_t s;
+  std::memcpy(&s, m._blob, sizeof(s));
+  return s;
+}
+inline void set_cu_meta(tensor_meta_t &m, const cu_shape_t &s) { std::memcpy(m._blob, &s, sizeof(s)); }
+
+// The header sits at base + kHeaderOffset so the payload starts at base + 512.
+constexpr size_t kHeaderOffset = 256 - (BBTS_META_BYTES - 256);  // 252
+constexpr size_t kPayloadOffset = kHeaderOffset + BBTS_META_BYTES;  // 512
+static_assert(kPayloadOffset % 256 == 0, "payload must be 256-byte aligned");
+
+// ---------------------------------------------------------------------------
+// Errors
+// ---------------------------------------------------------------------------
+void set_error(const char *fmt, ...);
+
+#define BBTS_HIP_CHECK(expr)                                                                 \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) {                                                                  \
+      ::bbts_hip::set_error("%s:%d: %s failed: %s", __FILE__, __LINE__, #expr,               \
+                            hipGetErrorString(e_));                                          \
+      return BBTS_E_HIP;                                                                     \
+    }                                                                                        \
+  } while (0)
+
+#define BBTS_REQUIRE(cond, code, ...)                                                        \
+  do {                                                                                       \
+    if (!(cond)) {                                                                           \
+      ::bbts_hip::set_error(__VA_ARGS__);                                                    \
+      return (code);                                                                         \
+    }                                                                                        \
+  } while (0)
+
+#define BBTS_LAUNCH_CHECK()                                                                  \
+  do {                                                                                       \
+    hipError_t e_ = hipGetLastError();                                                       \
+    if (e_ != hipSuccess) {                                                                  \
+      ::bbts_hip::set_error("%s:%d: kernel launch failed: %s", __FILE__, __LINE__,           \
+                            hipGetErrorString(e_));                                          \
+      return BBTS_E_HIP;                                                                     \
+    }                                                                                        \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Context: device, block registry (host mirrors of headers), HBM pool.
+// ---------------------------------------------------------------------------
+struct block_rec {
+  void *base = nullptr;      // allocation base (nullptr for registered external memory)
+  size_t bytes = 0;          // tensor bytes (header + payload)
+  size_t alloc_bytes = 0;    // bytes of the pooled allocation
+  tensor_meta_t meta;        // host mirror of the header
+};
+
+struct tune_key {
+  int M, N, K, dtype, precision;
+  bool operator<(const tune_key &o) const {
+    if (M != o.M) return M < o.M;
+    if (N != o.N) return N < o.N;
+    if (K != o.K) return K < o.K;
+    if (dtype != o.dtype) return dtype < o.dtype;
+    return precision < o.precision;
+  }
+};
+struct tune_val {
+  int cfg;
+  float us;
+};
+
+}  // namespace bbts_hip
+
+struct bbts_hip_ctx {
+  int device = 0;
+  int num_cus = 0;
+  int num_xcds = 8;
+  size_t hbm_bytes = 0;
+  std::mutex mu;
+  std::unordered_map<uintptr_t, bbts_hip::block_rec> blocks;
+  std::multimap<size_t, void *> free_pool;  // rounded size -> base
+  size_t pool_in_use = 0, pool_cached = 0;
+  std::map<bbts_hip::tune_key, bbts_hip::tune_val> tune_table;
+  // small device scratch used by kernels that need work counters (one per stream use)
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
+};
+
+namespace bbts_hip {
+
+// Set the context's device current on the calling thread.
+int bind(bbts_hip_ctx *ctx);
+// Look up a block; fills a copy of its record.
+int lookup(bbts_hip_ctx *ctx, const bbts_tensor_t *blk, block_rec *out);
+// Update the host mirror of a block's header.
+int update_meta(bbts_hip_ctx *ctx, const bbts_tensor_t *blk, const tensor_meta_t &m);
+// Pooled device allocation.
+int pool_alloc(bbts_hip_ctx *ctx, size_t bytes, void **ptr, size_t *rounded);
+void pool_release(bbts_hip_ctx *ctx, void *ptr, size_t rounded);
+
+inline const float *payload(const bbts_tensor_t *blk) {
+  return reinterpret_cast<const float *>(reinterpret_cast<const char *>(blk) + BBTS_META_BYTES);
+}
+inline float *payload(bbts_tensor_t *blk) {
+  return reinterpret_cast<float *>(reinterpret_cast<char *>(blk) + BBTS_META_BYTES);
+}
+inline uint32_t *meta_words(bbts_tensor_t *blk) {
+  // address of _blob[0] in device memory (4-byte aligned: header is at base+252)
+  return reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(blk) + 4);
+}
+
+// Internal launchers shared between translation units (stream_kernels.hip, gemm_*.hip).
+int launch_add(const float *a, const float *b, float *out, size_t n, uint32_t *meta_dst, uint32_t w0,
+               uint32_t w1, hipStream_t s);
+int launch_reduce(const float *const *in, int k, float *out, size_t n, uint32_t *meta_dst, uint32_t w0,
+                  uint32_t w1, hipStream_t s);
+int launch_sum_slices(const float *in, int64_t slice, int64_t nslices, float *out, hipStream_t s);
+int launch_permute(const float *in, float *out, int rank, const int64_t *dims, const int *perm, hipStream_t s);
+// FP32-output GEMM on strided operands; see gemm_f32.hip.
+struct gemm_args {
+  int M, N, K, batch;
+  const float *A;
+  int64_t lda, sa;
+  int ta;  // 0: A[m*lda+k]; 1: A[k*lda+m]
+  const float *B;
+  int64_t ldb, sb;
+  int tb;  // 0: B[k*ldb+n]; 1: B[n*ldb+k]
+  float *C;
+  int64_t ldc, sc;
+  float alpha;
+  uint32_t *meta_dst;
+  uint32_t w0, w1;
+};
+int launch_sgemm_f32(bbts_hip_ctx *ctx, const gemm_args &g, hipStream_t s);
+int launch_gemm_split_bf16(bbts_hip_ctx *ctx, const gemm_args &g, int slices_a, int slices_b, int terms,
+                           hipStream_t s);
+int launch_gemm_fp8(bbts_hip_ctx *ctx, const gemm_args &g, int refine, hipStream_t s);
+int launch_gemm_any(bbts_hip_ctx *ctx, const gemm_args &g, int precision, hipStream_t s);
+
+}  // namespace bbts_hip

@@ -1,0 +1,354 @@
+// Einsum-style contraction of two column-major tensors of rank <= 4: the
+// MI355X replacement of the cutensor library's contraction ud
+// (libraries/cutensor/contraction.h: parse :25-56, set_out_meta :58-63,
+// cpu_op::get_plan :267-377, cpu_op::operator() :414-469, gpu_op :596-624).
+//
+// Semantics: out[o] = alpha * sum over j, l, r of lhs[l] * rhs[r] for every
+// incident index tuple (the reference's own checker, reference() :131-202).
+// Modes are classified as in get_plan: i (lhs+out), j (lhs+rhs), k (rhs+out),
+// b (all three), l (lhs only), r (rhs only).
+//
+// Execution: each operand whose modes already form contiguous i|j|b (or
+// j|i|b) groups in the canonical group orders is consumed in place by the
+// batched GEMM through its leading dimension and transpose flag -- no
+// transpose pass.  Only an operand whose modes are interleaved, or that has
+// l / r modes to sum out, is first rewritten by one permute pass (and one
+// slice-sum pass).  The batched GEMM then runs as a row-major product
+// out^T = rhs'^T · lhs'^T (column-major [i,k] == row-major [k,i]).
+#include "common.h"
+
+namespace bbts_hip {
+
+namespace {
+
+enum { MI = 0, MJ, MK, ML, MR, MB };
+
+struct cinfo {
+  int nmodes = 0;
+  int64_t dims[16] = {0};
+  int rl = 0, rr = 0, ro = 0;
+  int ml[4] = {0}, mr[4] = {0}, mo[4] = {0};
+  float alpha = 1.0f;
+};
+
+int parse(const int32_t *params, int nparams, int rank_l, const int64_t *dims_l, int rank_r, const int64_t *dims_r,
+          cinfo *c) {
+  BBTS_REQUIRE(params && nparams >= 3, BBTS_E_INVALID, "contraction needs at least 3 params, got %d", nparams);
+  int rl = params[0], rr = params[1], ro = params[2];
+  BBTS_REQUIRE(rl >= 0 && rr >= 0 && ro >= 0 && rl <= 4 && rr <= 4 && ro <= 4, BBTS_E_INVALID,
+               "ranks (%d,%d,%d) must be in [0,4]", rl, rr, ro);
+  BBTS_REQUIRE(nparams >= 3 + rl + rr + ro + 1, BBTS_E_INVALID, "params list too short (%d)", nparams);
+  BBTS_REQUIRE(rl == rank_l && rr == rank_r, BBTS_E_SHAPE, "param ranks (%d,%d) differ from tensor ranks (%d,%d)", rl,
+               rr, rank_l, rank_r);
+  c->rl = rl; c->rr = rr; c->ro = ro;
+  int m = 0, i = 3;
+  for (int x = 0; x < rl; ++x, ++i) {
+    BBTS_REQUIRE(params[i] >= 0 && params[i] < 16, BBTS_E_INVALID, "mode id %d out of range", params[i]);
+    c->ml[x] = params[i];
+    m = std::max(m, 1 + params[i]);
+  }
+  for (int x = 0; x < rr; ++x, ++i) {
+    BBTS_REQUIRE(params[i] >= 0 && params[i] < 16, BBTS_E_INVALID, "mode id %d out of range", params[i]);
+    c->mr[x] = params[i];
+    m = std::max(m, 1 + params[i]);
+  }
+  for (int x = 0; x < ro; ++x, ++i) {
+    BBTS_REQUIRE(params[i] >= 0 && params[i] < 16, BBTS_E_INVALID, "mode id %d out of range", params[i]);
+    c->mo[x] = params[i];
+  }
+  std::memcpy(&c->alpha, &params[i], 4);
+  c->nmodes = m;
+  for (int x = 0; x < rl; ++x) c->dims[c->ml[x]] = dims_l[x];
+  for (int x = 0; x < rr; ++x) {
+    // a mode shared by both inputs must agree (the reference takes the rhs value)
+    BBTS_REQUIRE(c->dims[c->mr[x]] == 0 || c->dims[c->mr[x]] == dims_r[x] ||
+                     std::find(c->ml, c->ml + rl, c->mr[x]) == c->ml + rl,
+                 BBTS_E_SHAPE, "mode %d has extent %lld in lhs and %lld in rhs", c->mr[x],
+                 (long long)c->dims[c->mr[x]], (long long)dims_r[x]);
+    c->dims[c->mr[x]] = dims_r[x];
+  }
+  for (int x = 0; x < ro; ++x)
+    BBTS_REQUIRE(c->mo[x] < m, BBTS_E_INVALID, "output mode %d appears in no input", c->mo[x]);
+  return BBTS_OK;
+}
+
+int mode_class(const cinfo &c, int m) {
+  bool l = std::find(c.ml, c.ml + c.rl, m) != c.ml + c.rl;
+  bool r = std::find(c.mr, c.mr + c.rr, m) != c.mr + c.rr;
+  bool o = std::find(c.mo, c.mo + c.ro, m) != c.mo + c.ro;
+  if (l && !r && o) return MI;
+  if (l && r && !o) return MJ;
+  if (!l && r && o) return MK;
+  if (l && !r && !o) return ML;
+  if (!l && r && !o) return MR;
+  if (l && r && o) return MB;
+  return -1;
+}
+
+// Position list of `want` modes inside a tensor's mode list, or false if absent.
+bool positions(const int *modes, int rank, const std::vector<int> &want, std::vector<int> &pos) {
+  pos.clear();
+  for (int m : want) {
+    int p = (int)(std::find(modes, modes + rank, m) - modes);
+    if (p == rank) return false;
+    pos.push_back(p);
+  }
+  return true;
+}
+
+// Is the tensor (modes, rank) exactly the concatenation g1|g2|g3 (each in the given order)?
+bool is_concat(const int *modes, int rank, const std::vector<int> &g1, const std::vector<int> &g2,
+               const std::vector<int> &g3) {
+  std::vector<int> cat(g1);
+  cat.insert(cat.end(), g2.begin(), g2.end());
+  cat.insert(cat.end(), g3.begin(), g3.end());
+  if ((int)cat.size() != rank) return false;
+  for (int x = 0; x < rank; ++x)
+    if (cat[(size_t)x] != modes[x]) return false;
+  return true;
+}
+
+int64_t prod(const cinfo &c, const std::vector<int> &g) {
+  int64_t p = 1;
+  for (int m : g) p *= c.dims[m];
+  return p;
+}
+
+// Rewrite a column-major tensor with modes `modes` into the order `target`
+// (a permutation of modes), returning a pooled buffer.
+int permute_into(bbts_hip_ctx *ctx, const cinfo &c, const float *src, const int *modes, int rank,
+                 const std::vector<int> &target, float **dst, void **base, size_t *rounded, hipStream_t s) {
+  int64_t dims[4];
+  int perm[4];
+  size_t n = 1;
+  for (int x = 0; x < rank; ++x) {
+    dims[x] = c.dims[modes[x]];
+    n *= (size_t)dims[x];
+  }
+  for (int x = 0; x < rank; ++x) perm[x] = (int)(std::find(modes, modes + rank, target[(size_t)x]) - modes);
+  int rc = pool_alloc(ctx, n * sizeof(float) + 16, base, rounded);
+  if (rc) return rc;
+  *dst = static_cast<float *>(*base);
+  return launch_permute(src, *dst, rank, dims, perm, s);
+}
+
+struct scratch {
+  bbts_hip_ctx *ctx;
+  std::vector<std::pair<void *, size_t>> bufs;
+  ~scratch() {
+    for (auto &b : bufs) pool_release(ctx, b.first, b.second);
+  }
+};
+
+int run(bbts_hip_ctx *ctx, const cinfo &c, const float *lhs, const float *rhs, float *out, int precision,
+        uint32_t *meta_dst, const tensor_meta_t *out_meta, hipStream_t s) {
+  std::vector<int> ig, jg, kg, bg, lg, rg;
+  std::vector<int> cls((size_t)c.nmodes, -1);
+  for (int m = 0; m < c.nmodes; ++m) cls[(size_t)m] = mode_class(c, m);
+  // only modes that occur somewhere matter; the reference throws on an unused id ("should not happen")
+  for (int m = 0; m < c.nmodes; ++m) {
+    bool used = std::find(c.ml, c.ml + c.rl, m) != c.ml + c.rl || std::find(c.mr, c.mr + c.rr, m) != c.mr + c.rr;
+    BBTS_REQUIRE(used, BBTS_E_INVALID, "mode id %d is not used by either input", m);
+    BBTS_REQUIRE(cls[(size_t)m] >= 0, BBTS_E_INVALID, "mode %d has no class", m);
+  }
+  // group orders as in get_plan: i, k, b from the output; j, l from lhs; r from rhs
+  for (int x = 0; x < c.ro; ++x) {
+    int m = c.mo[x];
+    if (cls[(size_t)m] == MI) ig.push_back(m);
+    else if (cls[(size_t)m] == MK) kg.push_back(m);
+    else if (cls[(size_t)m] == MB) bg.push_back(m);
+  }
+  for (int x = 0; x < c.rl; ++x) {
+    int m = c.ml[x];
+    if (cls[(size_t)m] == MJ) jg.push_back(m);
+    else if (cls[(size_t)m] == ML) lg.push_back(m);
+  }
+  for (int x = 0; x < c.rr; ++x)
+    if (cls[(size_t)c.mr[x]] == MR) rg.push_back(c.mr[x]);
+  const int64_t ni = prod(c, ig), nj = prod(c, jg), nk = prod(c, kg), nb = prod(c, bg);
+  const int64_t nl = prod(c, lg), nr = prod(c, rg);
+  BBTS_REQUIRE(ni < (1LL << 31) && nj < (1LL << 31) && nk < (1LL << 31) && nb <= 65535, BBTS_E_UNSUPPORTED,
+               "contraction group extents exceed the GEMM limits");
+
+  scratch sc{ctx, {}};
+  int rc;
+  // ---- lhs as [i,j,b] (t_l = 0) or [j,i,b] (t_l = 1), l summed out
+  const float *L = lhs;
+  int t_l = 0;
+  if (is_concat(c.ml, c.rl, ig, jg, bg)) {
+    t_l = 0;
+  } else if (is_concat(c.ml, c.rl, jg, ig, bg)) {
+    t_l = 1;
+  } else {
+    std::vector<int> target(ig);
+    target.insert(target.end(), jg.begin(), jg.end());
+    target.insert(target.end(), bg.begin(), bg.end());
+    target.insert(target.end(), lg.begin(), lg.end());
+    float *buf;
+    void *base;
+    size_t r;
+    bool identity = is_concat(c.ml, c.rl, target, {}, {});
+    if (!identity) {
+      if ((rc = permute_into(ctx, c, lhs, c.ml, c.rl, target, &buf, &base, &r, s))) return rc;
+      sc.bufs.push_back({base, r});
+      L = buf;
+    }
+    if (nl > 1) {
+      void *b2;
+      size_t r2;
+      if ((rc = pool_alloc(ctx, sizeof(float) * (size_t)(ni * nj * nb) + 16, &b2, &r2))) return rc;
+      sc.bufs.push_back({b2, r2});
+      if ((rc = launch_sum_slices(L, ni * nj * nb, nl, static_cast<float *>(b2), s))) return rc;
+      L = static_cast<const float *>(b2);
+    }
+    t_l = 0;
+  }
+  // ---- rhs as [j,k,b] (t_r = 0) or [k,j,b] (t_r = 1), r summed out
+  const float *R = rhs;
+  int t_r = 0;
+  if (is_concat(c.mr, c.rr, jg, kg, bg)) {
+    t_r = 0;
+  } else if (is_concat(c.mr, c.rr, kg, jg, bg)) {
+    t_r = 1;
+  } else {
+    std::vector<int> target(jg);
+    target.insert(target.end(), kg.begin(), kg.end());
+    target.insert(target.end(), bg.begin(), bg.end());
+    target.insert(target.end(), rg.begin(), rg.end());
+    float *buf;
+    void *base;
+    size_t r;
+    bool identity = is_concat(c.mr, c.rr, target, {}, {});
+    if (!identity) {
+      if ((rc = permute_into(ctx, c, rhs, c.mr, c.rr, target, &buf, &base, &r, s))) return rc;
+      sc.bufs.push_back({base, r});
+      R = buf;
+    }
+    if (nr > 1) {
+      void *b2;
+      size_t r2;
+      if ((rc = pool_alloc(ctx, sizeof(float) * (size_t)(nj * nk * nb) + 16, &b2, &r2))) return rc;
+      sc.bufs.push_back({b2, r2});
+      if ((rc = launch_sum_slices(R, nj * nk * nb, nr, static_cast<float *>(b2), s))) return rc;
+      R = static_cast<const float *>(b2);
+    }
+    t_r = 0;
+  }
+  // ---- output as [i,k,b] (t_o = 0) or [k,i,b] (t_o = 1); anything else via a temporary + permute
+  int t_o = 0;
+  float *O = out;
+  bool out_direct = true;
+  if (is_concat(c.mo, c.ro, ig, kg, bg)) {
+    t_o = 0;
+  } else if (is_concat(c.mo, c.ro, kg, ig, bg)) {
+    t_o = 1;
+  } else {
+    out_direct = false;
+    void *b2;
+    size_t r2;
+    if ((rc = pool_alloc(ctx, sizeof(float) * (size_t)(ni * nk * nb) + 16, &b2, &r2))) return rc;
+    sc.bufs.push_back({b2, r2});
+    O = static_cast<float *>(b2);
+    t_o = 0;
+  }
+
+  // ---- batched GEMM (row-major view)
+  // L: t_l=0 -> col-major [ni,nj] = row-major [nj,ni];  t_l=1 -> row-major [ni,nj]
+  // R: t_r=0 -> col-major [nj,nk] = row-major [nk,nj];  t_r=1 -> row-major [nj,nk]
+  gemm_args g{};
+  g.batch = (int)nb;
+  g.alpha = c.alpha;
+  g.meta_dst = out_direct ? meta_dst : nullptr;
+  if (out_meta) {
+    dense_m_t w;
+    std::memcpy(&w, out_meta->_blob, sizeof(w));
+    g.w0 = w.num_rows;
+    g.w1 = w.num_cols;
+  }
+  if (t_o == 0) {
+    // out col-major [ni,nk] == row-major [nk, ni]:  C(nk x ni) = R^T(nk x nj) · L^T(nj x ni)
+    g.M = (int)nk; g.N = (int)ni; g.K = (int)nj;
+    g.A = R; g.sa = nj * nk;
+    if (t_r == 0) { g.ta = 0; g.lda = nj; } else { g.ta = 1; g.lda = nk; }
+    g.B = L; g.sb = ni * nj;
+    if (t_l == 0) { g.tb = 0; g.ldb = ni; } else { g.tb = 1; g.ldb = nj; }
+    g.C = O; g.ldc = ni; g.sc = ni * nk;
+  } else {
+    // out col-major [nk,ni] == row-major [ni, nk]:  C(ni x nk) = L(ni x nj) · R(nj x nk)
+    g.M = (int)ni; g.N = (int)nk; g.K = (int)nj;
+    g.A = L; g.sa = ni * nj;
+    if (t_l == 0) { g.ta = 1; g.lda = ni; } else { g.ta = 0; g.lda = nj; }
+    g.B = R; g.sb = nj * nk;
+    if (t_r == 0) { g.tb = 1; g.ldb = nj; } else { g.tb = 0; g.ldb = nk; }
+    g.C = O; g.ldc = nk; g.sc = ni * nk;
+  }
+  if (g.lda < 1) g.lda = 1;
+  if (g.ldb < 1) g.ldb = 1;
+  if (g.ldc < 1) g.ldc = 1;
+  if (ni * nk * nb > 0) {
+    if ((rc = launch_gemm_any(ctx, g, precision, s))) return rc;
+  }
+  if (!out_direct) {
+    // temporary is [i..., k..., b...] in canonical group order; permute into the output's order
+    std::vector<int> tmp_modes(ig);
+    tmp_modes.insert(tmp_modes.end(), kg.begin(), kg.end());
+    tmp_modes.insert(tmp_modes.end(), bg.begin(), bg.end());
+    int64_t dims[4];
+    int perm[4];
+    for (int x = 0; x < c.ro; ++x) dims[x] = c.dims[tmp_modes[(size_t)x]];
+    for (int x = 0; x < c.ro; ++x)
+      perm[x] = (int)(std::find(tmp_modes.begin(), tmp_modes.end(), c.mo[x]) - tmp_modes.begin());
+    if ((rc = launch_permute(O, out, c.ro, dims, perm, s))) return rc;
+  }
+  if (!sc.bufs.empty()) BBTS_HIP_CHECK(hipStreamSynchronize(s));
+  return BBTS_OK;
+}
+
+}  // namespace
+}  // namespace bbts_hip
+
+using namespace bbts_hip;
+
+extern "C" {
+
+int bbts_hip_contraction_raw(bbts_hip_ctx *ctx, const int32_t *params, int nparams, int rank_l, const int64_t *dims_l,
+                             const float *lhs, int rank_r, const int64_t *dims_r, const float *rhs, float *out,
+                             int precision, bbts_stream_t stream) {
+  int rc = bind(ctx);
+  if (rc) return rc;
+  cinfo c;
+  if ((rc = parse(params, nparams, rank_l, dims_l, rank_r, dims_r, &c))) return rc;
+  return run(ctx, c, lhs, rhs, out, precision, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int bbts_hip_contraction(bbts_hip_ctx *ctx, const bbts_tensor_t *lhs, const bbts_tensor_t *rhs, bbts_tensor_t *out,
+                         const int32_t *params, int nparams, int precision, bbts_stream_t stream) {
+  int rc = bind(ctx);
+  if (rc) return rc;
+  block_rec rl, rr, ro;
+  if ((rc = lookup(ctx, lhs, &rl)) || (rc = lookup(ctx, rhs, &rr)) || (rc = lookup(ctx, out, &ro))) return rc;
+  cu_shape_t sl = cu_meta(rl.meta), sr = cu_meta(rr.meta);
+  BBTS_REQUIRE(sl.rank <= 4 && sr.rank <= 4, BBTS_E_FORMAT, "cutensor rank above 4");
+  cinfo c;
+  if ((rc = parse(params, nparams, (int)sl.rank, sl.dims, (int)sr.rank, sr.dims, &c))) return rc;
+  // set_out_meta (:58-63)
+  cu_shape_t so{};
+  so.rank = (uint32_t)c.ro;
+  size_t n = 1;
+  for (int x = 0; x < c.ro; ++x) {
+    so.dims[x] = c.dims[c.mo[x]];
+    n *= (size_t)so.dims[x];
+  }
+  BBTS_REQUIRE(BBTS_META_BYTES + 4 * n <= ro.bytes, BBTS_E_SHAPE, "output block of %zu bytes cannot hold %zu elements",
+               ro.bytes, n);
+  tensor_meta_t mo = ro.meta;
+  set_cu_meta(mo, so);
+  if ((rc = update_meta(ctx, out, mo))) return rc;
+  // the shape bytes (40) land on the device header before the kernels run
+  BBTS_HIP_CHECK(hipMemcpyAsync(reinterpret_cast<char *>(out) + 4, mo._blob, sizeof(cu_shape_t),
+                                hipMemcpyHostToDevice, (hipStream_t)stream));
+  BBTS_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  return run(ctx, c, payload(lhs), payload(rhs), payload(out), precision, nullptr, nullptr, (hipStream_t)stream);
+}
+
+}  // extern "C"

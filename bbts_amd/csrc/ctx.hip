@@ -1,0 +1,316 @@
+// Context, error reporting, block registry and the pooled HBM allocator.
+//
+// Storage semantics follow the reference's in-memory storage
+// (src/storage/memory_storage.cc:70-103): one allocation per block holding
+// header + payload, freed when the block is removed.  Unlike the reference's
+// cudaMallocManaged-per-block, blocks come from a size-class pool of device
+// memory so the runner's per-op allocations (local_reduce_op.cc:56, one
+// temporary per step) never reach hipMalloc on the hot path, and the header
+// is placed so that the payload is 256-byte aligned for dwordx4 access.
+#include "common.h"
+
+namespace bbts_hip {
+
+static thread_local char g_last_error[1024] = "";
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+  va_end(ap);
+}
+
+int bind(bbts_hip_ctx *ctx) {
+  BBTS_REQUIRE(ctx != nullptr, BBTS_E_INVALID, "null context");
+  int cur = -1;
+  BBTS_HIP_CHECK(hipGetDevice(&cur));
+  if (cur != ctx->device) BBTS_HIP_CHECK(hipSetDevice(ctx->device));
+  return BBTS_OK;
+}
+
+int lookup(bbts_hip_ctx *ctx, const bbts_tensor_t *blk, block_rec *out) {
+  BBTS_REQUIRE(blk != nullptr, BBTS_E_INVALID, "null block");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  auto it = ctx->blocks.find(reinterpret_cast<uintptr_t>(blk));
+  BBTS_REQUIRE(it != ctx->blocks.end(), BBTS_E_UNKNOWN_BLOCK, "pointer %p is not a block of this context",
+               (const void *)blk);
+  *out = it->second;
+  return BBTS_OK;
+}
+
+int update_meta(bbts_hip_ctx *ctx, const bbts_tensor_t *blk, const tensor_meta_t &m) {
+  std::lock_guard<std::mutex> g(ctx->mu);
+  auto it = ctx->blocks.find(reinterpret_cast<uintptr_t>(blk));
+  BBTS_REQUIRE(it != ctx->blocks.end(), BBTS_E_UNKNOWN_BLOCK, "pointer %p is not a block of this context",
+               (const void *)blk);
+  it->second.meta = m;
+  return BBTS_OK;
+}
+
+// Size classes: 64 KiB granularity below 64 MiB, 2 MiB above.
+static size_t round_size(size_t bytes) {
+  const size_t small = 64u << 10, big = 2u << 20;
+  if (bytes < (64u << 20)) return (bytes + small - 1) / small * small;
+  return (bytes + big - 1) / big * big;
+}
+
+int pool_alloc(bbts_hip_ctx *ctx, size_t bytes, void **ptr, size_t *rounded) {
+  size_t r = round_size(bytes);
+  {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    auto it = ctx->free_pool.find(r);
+    if (it != ctx->free_pool.end()) {
+      *ptr = it->second;
+      ctx->free_pool.erase(it);
+      ctx->pool_cached -= r;
+      ctx->pool_in_use += r;
+      *rounded = r;
+      return BBTS_OK;
+    }
+  }
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, r);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    // release the cache and retry once
+    std::vector<void *> drop;
+    {
+      std::lock_guard<std::mutex> g(ctx->mu);
+      for (auto &kv : ctx->free_pool) drop.push_back(kv.second);
+      ctx->free_pool.clear();
+      ctx->pool_cached = 0;
+    }
+    BBTS_HIP_CHECK(hipDeviceSynchronize());
+    for (void *q : drop) (void)hipFree(q);
+    e = hipMalloc(&p, r);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("hipMalloc(%zu) failed: out of device memory", r);
+      return BBTS_E_OOM;
+    }
+  }
+  std::lock_guard<std::mutex> g(ctx->mu);
+  ctx->pool_in_use += r;
+  *ptr = p;
+  *rounded = r;
+  return BBTS_OK;
+}
+
+void pool_release(bbts_hip_ctx *ctx, void *ptr, size_t rounded) {
+  std::lock_guard<std::mutex> g(ctx->mu);
+  ctx->free_pool.emplace(rounded, ptr);
+  ctx->pool_in_use -= rounded;
+  ctx->pool_cached += rounded;
+}
+
+}  // namespace bbts_hip
+
+using namespace bbts_hip;
+
+extern "C" {
+
+const char *bbts_hip_strerror(int code) {
+  switch (code) {
+    case BBTS_OK: return "ok";
+    case BBTS_E_INVALID: return "invalid argument";
+    case BBTS_E_SHAPE: return "shape mismatch";
+    case BBTS_E_UNKNOWN_BLOCK: return "unknown block";
+    case BBTS_E_OOM: return "out of device memory";
+    case BBTS_E_HIP: return "HIP runtime error";
+    case BBTS_E_UNSUPPORTED: return "unsupported";
+    case BBTS_E_FORMAT: return "wrong tensor format";
+    default: return "unknown error";
+  }
+}
+
+const char *bbts_hip_last_error(void) { return g_last_error; }
+
+int bbts_hip_init(int device, bbts_hip_ctx **out) {
+  BBTS_REQUIRE(out != nullptr, BBTS_E_INVALID, "null output pointer");
+  int n = 0;
+  BBTS_HIP_CHECK(hipGetDeviceCount(&n));
+  BBTS_REQUIRE(device >= 0 && device < n, BBTS_E_INVALID, "device %d out of range (%d devices)", device, n);
+  BBTS_HIP_CHECK(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  BBTS_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+  auto *ctx = new bbts_hip_ctx();
+  ctx->device = device;
+  ctx->num_cus = prop.multiProcessorCount;
+  ctx->hbm_bytes = prop.totalGlobalMem;
+  ctx->num_xcds = 8;
+  ctx->scratch_bytes = 1u << 20;
+  hipError_t e = hipMalloc(&ctx->scratch, ctx->scratch_bytes);
+  if (e != hipSuccess) {
+    delete ctx;
+    set_error("scratch allocation failed: %s", hipGetErrorString(e));
+    return BBTS_E_OOM;
+  }
+  e = hipMemset(ctx->scratch, 0, ctx->scratch_bytes);
+  if (e != hipSuccess) {
+    (void)hipFree(ctx->scratch);
+    delete ctx;
+    set_error("scratch clear failed: %s", hipGetErrorString(e));
+    return BBTS_E_HIP;
+  }
+  *out = ctx;
+  return BBTS_OK;
+}
+
+int bbts_hip_shutdown(bbts_hip_ctx *ctx) {
+  if (!ctx) return BBTS_OK;
+  int rc = bind(ctx);
+  if (rc) return rc;
+  BBTS_HIP_CHECK(hipDeviceSynchronize());
+  for (auto &kv : ctx->blocks)
+    if (kv.second.base) (void)hipFree(kv.second.base);
+  for (auto &kv : ctx->free_pool) (void)hipFree(kv.second);
+  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  delete ctx;
+  return BBTS_OK;
+}
+
+int bbts_hip_sync(bbts_hip_ctx *ctx, bbts_stream_t stream) {
+  int rc = bind(ctx);
+  if (rc) return rc;
+  BBTS_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  return BBTS_OK;
+}
+
+int bbts_hip_device_info(bbts_hip_ctx *ctx, int *num_cus, int *num_xcds, size_t *hbm_bytes) {
+  BBTS_REQUIRE(ctx, BBTS_E_INVALID, "null context");
+  if (num_cus) *num_cus = ctx->num_cus;
+  if (num_xcds) *num_xcds = ctx->num_xcds;
+  if (hbm_bytes) *hbm_bytes = ctx->hbm_bytes;
+  return BBTS_OK;
+}
+
+size_t bbts_hip_dense_size(uint32_t rows, uint32_t cols) {
+  return (size_t)BBTS_META_BYTES + (size_t)rows * (size_t)cols * sizeof(float);
+}
+
+int bbts_hip_block_alloc(bbts_hip_ctx *ctx, const void *meta, size_t bytes, bbts_tensor_t **blk) {
+  int rc = bind(ctx);
+  if (rc) return rc;
+  BBTS_REQUIRE(meta && blk, BBTS_E_INVALID, "null meta or output");
+  BBTS_REQUIRE(bytes >= BBTS_META_BYTES, BBTS_E_INVALID, "block of %zu bytes is smaller than its header", bytes);
+  void *base = nullptr;
+  size_t rounded = 0;
+  rc = pool_alloc(ctx, bytes + kHeaderOffset, &base, &rounded);
+  if (rc) return rc;
+  char *hdr = static_cast<char *>(base) + kHeaderOffset;
+  hipError_t e = hipMemcpy(hdr, meta, BBTS_META_BYTES, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    pool_release(ctx, base, rounded);
+    set_error("header upload failed: %s", hipGetErrorString(e));
+    return BBTS_E_HIP;
+  }
+  block_rec r;
+  r.base = base;
+  r.bytes = bytes;
+  r.alloc_bytes = rounded;
+  std::memcpy(&r.meta, meta, BBTS_META_BYTES);
+  {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->blocks[reinterpret_cast<uintptr_t>(hdr)] = r;
+  }
+  *blk = reinterpret_cast<bbts_tensor_t *>(hdr);
+  return BBTS_OK;
+}
+
+int bbts_hip_block_register(bbts_hip_ctx *ctx, void *hdr, size_t bytes, const void *meta, bbts_tensor_t **blk) {
+  BBTS_REQUIRE(ctx && hdr && meta && blk, BBTS_E_INVALID, "null argument");
+  BBTS_REQUIRE(bytes >= BBTS_META_BYTES, BBTS_E_INVALID, "block of %zu bytes is smaller than its header", bytes);
+  BBTS_REQUIRE(reinterpret_cast<uintptr_t>(hdr) % 4 == 0, BBTS_E_INVALID, "header must be 4-byte aligned");
+  block_rec r;
+  r.base = nullptr;
+  r.bytes = bytes;
+  r.alloc_bytes = 0;
+  std::memcpy(&r.meta, meta, BBTS_META_BYTES);
+  std::lock_guard<std::mutex> g(ctx->mu);
+  ctx->blocks[reinterpret_cast<uintptr_t>(hdr)] = r;
+  *blk = reinterpret_cast<bbts_tensor_t *>(hdr);
+  return BBTS_OK;
+}
+
+int bbts_hip_block_free(bbts_hip_ctx *ctx, bbts_tensor_t *blk) {
+  BBTS_REQUIRE(ctx && blk, BBTS_E_INVALID, "null argument");
+  block_rec r;
+  {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    auto it = ctx->blocks.find(reinterpret_cast<uintptr_t>(blk));
+    BBTS_REQUIRE(it != ctx->blocks.end(), BBTS_E_UNKNOWN_BLOCK, "pointer %p is not a block of this context",
+                 (void *)blk);
+    r = it->second;
+    ctx->blocks.erase(it);
+  }
+  // Stream-ordered reuse: the pool hands the memory out again only to later
+  // work; kernels still in flight on other streams must be synchronised by the
+  // caller before freeing (as the reference's storage frees after retirement).
+  if (r.base) pool_release(ctx, r.base, r.alloc_bytes);
+  return BBTS_OK;
+}
+
+int bbts_hip_block_meta(bbts_hip_ctx *ctx, const bbts_tensor_t *blk, void *meta_out, size_t *bytes_out) {
+  BBTS_REQUIRE(ctx, BBTS_E_INVALID, "null context");
+  block_rec r;
+  int rc = lookup(ctx, blk, &r);
+  if (rc) return rc;
+  if (meta_out) std::memcpy(meta_out, &r.meta, BBTS_META_BYTES);
+  if (bytes_out) *bytes_out = r.bytes;
+  return BBTS_OK;
+}
+
+int bbts_hip_block_set_meta(bbts_hip_ctx *ctx, bbts_tensor_t *blk, const void *meta, bbts_stream_t stream) {
+  int rc = bind(ctx);
+  if (rc) return rc;
+  BBTS_REQUIRE(meta, BBTS_E_INVALID, "null meta");
+  tensor_meta_t m;
+  std::memcpy(&m, meta, BBTS_META_BYTES);
+  rc = update_meta(ctx, blk, m);
+  if (rc) return rc;
+  BBTS_HIP_CHECK(hipMemcpyAsync(blk, meta, BBTS_META_BYTES, hipMemcpyHostToDevice, (hipStream_t)stream));
+  BBTS_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  return BBTS_OK;
+}
+
+void *bbts_hip_block_data(const bbts_tensor_t *blk) {
+  return const_cast<float *>(payload(blk));
+}
+
+int bbts_hip_block_upload(bbts_hip_ctx *ctx, bbts_tensor_t *blk, const void *host, size_t bytes,
+                          bbts_stream_t stream) {
+  int rc = bind(ctx);
+  if (rc) return rc;
+  block_rec r;
+  rc = lookup(ctx, blk, &r);
+  if (rc) return rc;
+  BBTS_REQUIRE(bytes + BBTS_META_BYTES <= r.bytes, BBTS_E_INVALID, "upload of %zu bytes exceeds payload %zu", bytes,
+               r.bytes - BBTS_META_BYTES);
+  BBTS_HIP_CHECK(hipMemcpyAsync(payload(blk), host, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+  BBTS_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  return BBTS_OK;
+}
+
+int bbts_hip_block_download(bbts_hip_ctx *ctx, const bbts_tensor_t *blk, void *host, size_t bytes,
+                            bbts_stream_t stream) {
+  int rc = bind(ctx);
+  if (rc) return rc;
+  block_rec r;
+  rc = lookup(ctx, blk, &r);
+  if (rc) return rc;
+  BBTS_REQUIRE(bytes <= r.bytes, BBTS_E_INVALID, "download of %zu bytes exceeds block %zu", bytes, r.bytes);
+  // bytes counts from the header: the whole tensor as the reference's broadcast sends it
+  BBTS_HIP_CHECK(hipMemcpyAsync(host, blk, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  BBTS_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  return BBTS_OK;
+}
+
+int bbts_hip_pool_stats(bbts_hip_ctx *ctx, size_t *in_use, size_t *cached) {
+  BBTS_REQUIRE(ctx, BBTS_E_INVALID, "null context");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  if (in_use) *in_use = ctx->pool_in_use;
+  if (cached) *cached = ctx->pool_cached;
+  return BBTS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,709 @@
+// BF16 / split-precision block GEMM: single-op launches and the grouped
+// persistent launch that runs a whole stream of uneven block GEMMs.
+//
+// Grouped launch: every (op, output tile) pair of the ops that share one tile
+// configuration becomes a work item; the planner orders the items (largest
+// predicted tile time first, i.e. LPT list scheduling) and a persistent grid
+// of min(items, CUs x occupancy) workgroups pulls items in that order from a
+// device ticket counter.  The counter resets itself: tickets are handed out
+// until every workgroup has drawn one past the end, so the workgroup that
+// draws the very last ticket (items + grid - 1) knows no further draw can
+// happen and zeroes it.  A captured hipGraph therefore replays with no memset
+// node and no host work.
+#include "gemm_bf16.cuh"
+
+namespace bbts_hip {
+
+using namespace bf16gemm;
+
+namespace {
+
+struct gop {
+  op_view v;
+  int tiles_n;
+  int pad;
+};
+
+__device__ inline void map_tile2(int lin, int tiles_m, int tiles_n, int *tm, int *tn) {
+  int total = tiles_m * tiles_n;
+  if (total % 8 == 0) {
+    int per = total / 8;
+    lin = (lin % 8) * per + lin / 8;
+  }
+  const int G = 8;
+  int width = G * tiles_n;
+  int group = lin / width;
+  int first_m = group * G;
+  int gsize = tiles_m - first_m < G ? tiles_m - first_m : G;
+  int in_group = lin % width;
+  *tm = first_m + in_group % gsize;
+  *tn = in_group / gsize;
+}
+
+template <int BM, int BN, typename InT, int S, int T>
+__global__ __launch_bounds__(256) void gemm_single_kernel(op_view op, int64_t sa, int64_t sb, int64_t sc, int tiles_m,
+                                                          int tiles_n, uint32_t *meta_dst, uint32_t w0, uint32_t w1) {
+  extern __shared__ __align__(16) uint16_t smem[];
+  if (meta_dst != nullptr && blockIdx.x == 0 && blockIdx.z == 0 && threadIdx.x == 0) {
+    meta_dst[0] = w0;
+    meta_dst[1] = w1;
+  }
+  int tm, tn;
+  map_tile2(blockIdx.x, tiles_m, tiles_n, &tm, &tn);
+  op_view o = op;
+  o.A = static_cast<const InT *>(op.A) + (int64_t)blockIdx.z * sa;
+  o.B = static_cast<const InT *>(op.B) + (int64_t)blockIdx.z * sb;
+  o.C = op.C + (int64_t)blockIdx.z * sc;
+  tile<BM, BN, InT, S, T>(o, tm * BM, tn * BN, smem);
+}
+
+template <int BM, int BN, typename InT, int S, int T>
+__global__ __launch_bounds__(256) void gemm_grouped_kernel(const gop *__restrict__ ops, const int2 *__restrict__ items,
+                                                           int nitems, unsigned *counter) {
+  extern __shared__ __align__(16) uint16_t smem[];
+  __shared__ int s_ticket;
+  for (;;) {
+    if (threadIdx.x == 0) s_ticket = (int)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int t = s_ticket;
+    __syncthreads();
+    if (t >= nitems) {
+      if (threadIdx.x == 0 && t == nitems + (int)gridDim.x - 1) atomicExch(counter, 0u);
+      return;
+    }
+    const int2 it = items[t];
+    const gop o = ops[it.x];
+    const int tm = it.y / o.tiles_n, tn = it.y % o.tiles_n;
+    tile<BM, BN, InT, S, T>(o.v, tm * BM, tn * BN, smem);
+  }
+}
+
+// ---- configuration table ---------------------------------------------------
+struct cfg_shape {
+  int bm, bn;
+};
+constexpr cfg_shape kCfgs[] = {{256, 128}, {128, 128}, {128, 64}, {64, 64}, {64, 128}};
+constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
+
+// variant: 0 = BF16 operands (S=T=1); 1,2,3 = FP32 operands split into 1,2,3 slices (T=S)
+typedef void (*single_fn)(op_view, int64_t, int64_t, int64_t, int, int, uint32_t *, uint32_t, uint32_t);
+typedef void (*grouped_fn)(const gop *, const int2 *, int, unsigned *);
+
+template <int BM, int BN>
+struct cfg_kernels {
+  static single_fn single(int variant) {
+    switch (variant) {
+      case 0: return gemm_single_kernel<BM, BN, uint16_t, 1, 1>;
+      case 1: return gemm_single_kernel<BM, BN, float, 1, 1>;
+      case 2: return gemm_single_kernel<BM, BN, float, 2, 2>;
+      default: return gemm_single_kernel<BM, BN, float, 3, 3>;
+    }
+  }
+  static grouped_fn grouped(int variant) {
+    switch (variant) {
+      case 0: return gemm_grouped_kernel<BM, BN, uint16_t, 1, 1>;
+      case 1: return gemm_grouped_kernel<BM, BN, float, 1, 1>;
+      case 2: return gemm_grouped_kernel<BM, BN, float, 2, 2>;
+      default: return gemm_grouped_kernel<BM, BN, float, 3, 3>;
+    }
+  }
+};
+
+int smem_bytes(int cfg, int variant) {
+  int S = variant == 0 ? 1 : variant;
+  return 2 * S * (kCfgs[cfg].bm + kCfgs[cfg].bn) * BK * 2;
+}
+
+single_fn get_single(int cfg, int variant) {
+  switch (cfg) {
+    case 0: return cfg_kernels<256, 128>::single(variant);
+    case 1: return cfg_kernels<128, 128>::single(variant);
+    case 2: return cfg_kernels<128, 64>::single(variant);
+    case 3: return cfg_kernels<64, 64>::single(variant);
+    default: return cfg_kernels<64, 128>::single(variant);
+  }
+}
+grouped_fn get_grouped(int cfg, int variant) {
+  switch (cfg) {
+    case 0: return cfg_kernels<256, 128>::grouped(variant);
+    case 1: return cfg_kernels<128, 128>::grouped(variant);
+    case 2: return cfg_kernels<128, 64>::grouped(variant);
+    case 3: return cfg_kernels<64, 64>::grouped(variant);
+    default: return cfg_kernels<64, 128>::grouped(variant);
+  }
+}
+
+// Raise the dynamic-LDS limit of a kernel once.
+std::mutex g_attr_mu;
+std::map<const void *, int> g_attr_done;
+int ensure_smem(const void *fn, int bytes) {
+  std::lock_guard<std::mutex> g(g_attr_mu);
+  auto it = g_attr_done.find(fn);
+  if (it != g_attr_done.end() && it->second >= bytes) return BBTS_OK;
+  BBTS_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  g_attr_done[fn] = bytes;
+  return BBTS_OK;
+}
+
+// Workgroups of one config that fit on a CU (LDS-limited; 160 KiB per CU).
+int occupancy(int cfg, int variant) {
+  int b = smem_bytes(cfg, variant) + 64;
+  int by_lds = (160 * 1024) / b;
+  // waves per SIMD the register budget allows (4 waves per workgroup, one per SIMD)
+  static const int kByRegs[] = {1, 2, 3, 4, 3};
+  int by_waves = kByRegs[cfg];
+  int o = by_lds < by_waves ? by_lds : by_waves;
+  return o < 1 ? 1 : o;
+}
+
+inline bool al16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+int variant_of(int dtype, int precision, int *variant) {
+  if (dtype == 1) {
+    *variant = 0;  // BF16 operands: one slice is exact
+    return BBTS_OK;
+  }
+  switch (precision) {
+    case BBTS_PREC_BF16X1: *variant = 1; return BBTS_OK;
+    case BBTS_PREC_BF16X3: *variant = 2; return BBTS_OK;
+    case BBTS_PREC_BF16X6: *variant = 3; return BBTS_OK;
+    case BBTS_PREC_AUTO: *variant = 3; return BBTS_OK;
+    default:
+      set_error("grouped GEMM on FP32 operands needs a BF16 split precision, got %d", precision);
+      return BBTS_E_UNSUPPORTED;
+  }
+}
+
+int heuristic_cfg(int M, int N, int K) {
+  (void)K;
+  long long t256 = (long long)((M + 255) / 256) * ((N + 127) / 128);
+  if (M >= 1024 && N >= 512 && t256 >= 32) return 0;
+  if (M >= 256 && N >= 256) return 1;
+  if (M >= 128 && N >= 64) return 2;
+  return 3;
+}
+
+}  // namespace
+
+int launch_gemm_split_bf16(bbts_hip_ctx *ctx, const gemm_args &g, int slices_a, int slices_b, int terms,
+                           hipStream_t s) {
+  BBTS_REQUIRE(g.M >= 0 && g.N >= 0 && g.K >= 0 && g.batch >= 1, BBTS_E_INVALID, "bad GEMM sizes");
+  if (g.M == 0 || g.N == 0) return BBTS_OK;
+  int S = slices_a < 0 ? 3 : slices_a;
+  (void)slices_b;
+  (void)terms;
+  int variant = S;  // FP32 operands split into S slices, T = S
+  // transposed operands: stage a row-major copy first (pooled scratch)
+  const float *A = g.A, *B = g.B;
+  int64_t lda = g.lda, ldb = g.ldb, sa = g.sa, sb = g.sb;
+  void *tmpA = nullptr, *tmpB = nullptr;
+  size_t rA = 0, rB = 0;
+  int rc = BBTS_OK;
+  if (g.ta || g.tb) {
+    BBTS_REQUIRE(g.batch == 1, BBTS_E_UNSUPPORTED, "batched split GEMM with transposed operands");
+    if (g.ta) {
+      BBTS_REQUIRE(g.lda == g.M, BBTS_E_UNSUPPORTED, "transposed A must be packed (lda == M)");
+      if ((rc = pool_alloc(ctx, sizeof(float) * (size_t)g.M * g.K + 16, &tmpA, &rA))) return rc;
+      int64_t dims[2] = {g.M, g.K};
+      int perm[2] = {1, 0};
+      // A stored K x M row-major == column-major [M, K]; row-major M x K == column-major [K, M]
+      if ((rc = launch_permute(g.A, static_cast<float *>(tmpA), 2, dims, perm, s))) return rc;
+      A = static_cast<const float *>(tmpA);
+      lda = g.K;
+    }
+    if (g.tb) {
+      BBTS_REQUIRE(g.ldb == g.K, BBTS_E_UNSUPPORTED, "transposed B must be packed (ldb == K)");
+      if ((rc = pool_alloc(ctx, sizeof(float) * (size_t)g.K * g.N + 16, &tmpB, &rB))) return rc;
+      int64_t dims[2] = {g.K, g.N};
+      int perm[2] = {1, 0};
+      if ((rc = launch_permute(g.B, static_cast<float *>(tmpB), 2, dims, perm, s))) return rc;
+      B = static_cast<const float *>(tmpB);
+      ldb = g.N;
+    }
+  }
+  int cfg = heuristic_cfg(g.M, g.N, g.K);
+  op_view v{};
+  v.A = A; v.B = B; v.C = g.C;
+  v.lda = lda; v.ldb = ldb; v.ldc = g.ldc;
+  v.M = g.M; v.N = g.N; v.K = g.K; v.alpha = g.alpha;
+  v.vecA = al16(A) && lda % 4 == 0 && (g.batch == 1 || sa % 4 == 0);
+  v.vecB = al16(B) && ldb % 4 == 0 && (g.batch == 1 || sb % 4 == 0);
+  int tiles_m = (g.M + kCfgs[cfg].bm - 1) / kCfgs[cfg].bm, tiles_n = (g.N + kCfgs[cfg].bn - 1) / kCfgs[cfg].bn;
+  single_fn fn = get_single(cfg, variant);
+  int bytes = smem_bytes(cfg, variant);
+  if ((rc = ensure_smem((const void *)fn, bytes))) return rc;
+  hipLaunchKernelGGL(fn, dim3((unsigned)(tiles_m * tiles_n), 1, (unsigned)g.batch), dim3(256), (unsigned)bytes, s, v,
+                     sa, sb, g.sc, tiles_m, tiles_n, g.meta_dst, g.w0, g.w1);
+  BBTS_LAUNCH_CHECK();
+  if (tmpA || tmpB) {
+    // the scratch copies are released once the stream has consumed them
+    BBTS_HIP_CHECK(hipStreamSynchronize(s));
+    if (tmpA) pool_release(ctx, tmpA, rA);
+    if (tmpB) pool_release(ctx, tmpB, rB);
+  }
+  return BBTS_OK;
+}
+
+}  // namespace bbts_hip
+
+// ===========================================================================
+// Batch (grouped stream) API
+// ===========================================================================
+using namespace bbts_hip;
+
+struct bbts_batch {
+  bbts_hip_ctx *ctx = nullptr;
+  std::vector<bbts_gemm_desc_t> ops;
+  std::vector<int> prec;
+  std::vector<int> cfg;        // planned config per op
+  std::vector<float> tile_us;  // planned per-tile cost per op (ordering key)
+  struct group {
+    int cfg = 0, variant = 0;
+    int nitems = 0, grid = 0;
+    double flops = 0.0;
+    size_t ops_off = 0, items_off = 0, counter_off = 0;
+  };
+  std::vector<group> groups;
+  void *dev = nullptr;
+  size_t dev_bytes = 0;
+  bool planned = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace bbts_hip {
+int costmodel_predict_host(bbts_costmodel *cm, const std::vector<float> &feats, int n, std::vector<float> &out);
+}
+
+namespace {
+
+void drop_graph(bbts_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+long long tiles_of(const bbts_gemm_desc_t &d, int cfg) {
+  return (long long)((d.M + kCfgs[cfg].bm - 1) / kCfgs[cfg].bm) * ((d.N + kCfgs[cfg].bn - 1) / kCfgs[cfg].bn);
+}
+
+op_view view_of(const bbts_gemm_desc_t &d) {
+  op_view v{};
+  v.A = d.A; v.B = d.B; v.C = d.C;
+  v.lda = d.lda; v.ldb = d.ldb; v.ldc = d.ldc;
+  v.M = d.M; v.N = d.N; v.K = d.K;
+  v.alpha = 1.0f;
+  int epv = d.dtype == 1 ? 8 : 4;
+  v.vecA = al16(d.A) && d.lda % epv == 0;
+  v.vecB = al16(d.B) && d.ldb % epv == 0;
+  return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bbts_hip_num_tile_configs(void) { return kNumCfgs; }
+
+int bbts_hip_tile_config_shape(int cfg, int *bm, int *bn, int *bk) {
+  BBTS_REQUIRE(cfg >= 0 && cfg < kNumCfgs, BBTS_E_INVALID, "config %d out of range", cfg);
+  if (bm) *bm = kCfgs[cfg].bm;
+  if (bn) *bn = kCfgs[cfg].bn;
+  if (bk) *bk = BK;
+  return BBTS_OK;
+}
+
+int bbts_hip_gemm_desc_launch(bbts_hip_ctx *ctx, const bbts_gemm_desc_t *d, int precision, int cfg,
+                              bbts_stream_t stream) {
+  int rc = bind(ctx);
+  if (rc) return rc;
+  BBTS_REQUIRE(d && cfg >= 0 && cfg < kNumCfgs, BBTS_E_INVALID, "bad descriptor or config");
+  BBTS_REQUIRE(d->M >= 0 && d->N >= 0 && d->K >= 0, BBTS_E_INVALID, "bad sizes");
+  if (d->M == 0 || d->N == 0) return BBTS_OK;
+  int variant;
+  if ((rc = variant_of(d->dtype, precision, &variant))) return rc;
+  op_view v = view_of(*d);
+  int tiles_m = (d->M + kCfgs[cfg].bm - 1) / kCfgs[cfg].bm, tiles_n = (d->N + kCfgs[cfg].bn - 1) / kCfgs[cfg].bn;
+  single_fn fn = get_single(cfg, variant);
+  int bytes = smem_bytes(cfg, variant);
+  if ((rc = ensure_smem((const void *)fn, bytes))) return rc;
+  hipLaunchKernelGGL(fn, dim3((unsigned)(tiles_m * tiles_n)), dim3(256), (unsigned)bytes, (hipStream_t)stream, v,
+                     (int64_t)0, (int64_t)0, (int64_t)0, tiles_m, tiles_n, (uint32_t *)nullptr, 0u, 0u);
+  BBTS_LAUNCH_CHECK();
+  return BBTS_OK;
+}
+
+int bbts_hip_batch_create(bbts_hip_ctx *ctx, bbts_batch **out) {
+  BBTS_REQUIRE(ctx && out, BBTS_E_INVALID, "null argument");
+  auto *b = new bbts_batch();
+  b->ctx = ctx;
+  *out = b;
+  return BBTS_OK;
+}
+
+int bbts_hip_batch_clear(bbts_batch *b) {
+  BBTS_REQUIRE(b, BBTS_E_INVALID, "null batch");
+  drop_graph(b);
+  b->ops.clear();
+  b->prec.clear();
+  b->cfg.clear();
+  b->tile_us.clear();
+  b->groups.clear();
+  b->planned = false;
+  return BBTS_OK;
+}
+
+int bbts_hip_batch_destroy(bbts_batch *b) {
+  if (!b) return BBTS_OK;
+  bind(b->ctx);
+  (void)hipDeviceSynchronize();
+  drop_graph(b);
+  if (b->dev) (void)hipFree(b->dev);
+  delete b;
+  return BBTS_OK;
+}
+
+int bbts_hip_batch_add_gemm(bbts_batch *b, const bbts_gemm_desc_t *d, int precision) {
+  BBTS_REQUIRE(b && d, BBTS_E_INVALID, "null argument");
+  BBTS_REQUIRE(d->M >= 0 && d->N >= 0 && d->K >= 0, BBTS_E_INVALID, "bad sizes");
+  BBTS_REQUIRE(d->dtype == 0 || d->dtype == 1, BBTS_E_INVALID, "dtype must be 0 (FP32) or 1 (BF16)");
+  BBTS_REQUIRE(d->lda >= d->K && d->ldb >= d->N && d->ldc >= d->N, BBTS_E_INVALID, "leading dimension too small");
+  BBTS_REQUIRE((d->M == 0 || d->N == 0) || (d->C && (d->K == 0 || (d->A && d->B))), BBTS_E_INVALID, "null operand");
+  int variant;
+  int rc = variant_of(d->dtype, precision, &variant);
+  if (rc) return rc;
+  drop_graph(b);
+  b->ops.push_back(*d);
+  b->prec.push_back(precision);
+  b->planned = false;
+  return BBTS_OK;
+}
+
+int bbts_hip_batch_op_config(bbts_batch *b, int op, int *cfg) {
+  BBTS_REQUIRE(b && cfg && op >= 0 && op < (int)b->cfg.size(), BBTS_E_INVALID, "bad op index");
+  *cfg = b->cfg[(size_t)op];
+  return BBTS_OK;
+}
+
+int bbts_hip_batch_plan(bbts_batch *b, int policy, bbts_costmodel *cm) {
+  BBTS_REQUIRE(b, BBTS_E_INVALID, "null batch");
+  int rc = bind(b->ctx);
+  if (rc) return rc;
+  drop_graph(b);
+  const size_t n = b->ops.size();
+  b->cfg.assign(n, 1);
+  b->tile_us.assign(n, 0.0f);
+  std::vector<long long> order_key_flops(n);
+  for (size_t i = 0; i < n; ++i) {
+    const auto &d = b->ops[i];
+    order_key_flops[i] = 2LL * d.M * d.N * d.K;
+  }
+  if (policy >= BBTS_SCHED_FORCED_CFG0 && policy < BBTS_SCHED_FORCED_CFG0 + kNumCfgs) {
+    for (size_t i = 0; i < n; ++i) b->cfg[i] = policy - BBTS_SCHED_FORCED_CFG0;
+  } else if (policy == BBTS_SCHED_FIFO) {
+    for (size_t i = 0; i < n; ++i) b->cfg[i] = 1;
+  } else if (policy == BBTS_SCHED_HEURISTIC) {
+    for (size_t i = 0; i < n; ++i) {
+      const auto &d = b->ops[i];
+      b->cfg[i] = heuristic_cfg(d.M, d.N, d.K);
+      // per-tile cost proxy: tile area x K
+      b->tile_us[i] = (float)kCfgs[b->cfg[i]].bm * kCfgs[b->cfg[i]].bn * (float)d.K * 1e-9f;
+    }
+  } else if (policy == BBTS_SCHED_TUNED) {
+    for (size_t i = 0; i < n; ++i) {
+      const auto &d = b->ops[i];
+      int cfg = -1;
+      float us = 0;
+      bbts_hip_tune_lookup(b->ctx, d.M, d.N, d.K, d.dtype, b->prec[i], &cfg, &us);
+      if (cfg < 0) {
+        cfg = heuristic_cfg(d.M, d.N, d.K);
+        us = (float)kCfgs[cfg].bm * kCfgs[cfg].bn * (float)d.K * 1e-6f;
+      } else {
+        us = us / (float)tiles_of(d, cfg);
+      }
+      b->cfg[i] = cfg;
+      b->tile_us[i] = us;
+    }
+  } else if (policy == BBTS_SCHED_LEARNED) {
+    BBTS_REQUIRE(cm != nullptr, BBTS_E_INVALID, "learned policy needs a cost model");
+    const int nf = bbts_hip_costmodel_num_features();
+    std::vector<float> feats(n * kNumCfgs * (size_t)nf);
+    for (size_t i = 0; i < n; ++i) {
+      const auto &d = b->ops[i];
+      for (int c = 0; c < kNumCfgs; ++c)
+        bbts_hip_costmodel_features(d.M, d.N, d.K, d.dtype, b->prec[i], c,
+                                    &feats[(i * kNumCfgs + (size_t)c) * (size_t)nf]);
+    }
+    std::vector<float> pred;
+    if ((rc = costmodel_predict_host(cm, feats, (int)(n * kNumCfgs), pred))) return rc;
+    for (size_t i = 0; i < n; ++i) {
+      int best = 0;
+      float best_us = 0;
+      for (int c = 0; c < kNumCfgs; ++c) {
+        float us = expf(pred[i * kNumCfgs + (size_t)c]);  // predicted grouped cost of the op, microseconds
+        if (c == 0 || us < best_us) {
+          best = c;
+          best_us = us;
+        }
+      }
+      b->cfg[i] = best;
+      b->tile_us[i] = best_us / (float)tiles_of(b->ops[i], best);
+    }
+  } else {
+    set_error("unknown schedule policy %d", policy);
+    return BBTS_E_INVALID;
+  }
+
+  // group ops by (cfg, variant); order the ops of a group by tile cost, largest first
+  std::map<std::pair<int, int>, std::vector<int>> gmap;
+  for (size_t i = 0; i < n; ++i) {
+    const auto &d = b->ops[i];
+    if (d.M == 0 || d.N == 0) continue;
+    int variant;
+    if ((rc = variant_of(d.dtype, b->prec[i], &variant))) return rc;
+    gmap[{b->cfg[i], variant}].push_back((int)i);
+  }
+  b->groups.clear();
+  size_t bytes = 0;
+  std::vector<std::vector<gop>> gops;
+  std::vector<std::vector<int2>> gitems;
+  for (auto &kv : gmap) {
+    auto &idx = kv.second;
+    if (policy != BBTS_SCHED_FIFO && policy < BBTS_SCHED_FORCED_CFG0) {
+      std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
+        if (b->tile_us[(size_t)x] != b->tile_us[(size_t)y]) return b->tile_us[(size_t)x] > b->tile_us[(size_t)y];
+        return order_key_flops[(size_t)x] > order_key_flops[(size_t)y];
+      });
+    }
+    bbts_batch::group grp;
+    grp.cfg = kv.first.first;
+    grp.variant = kv.first.second;
+    std::vector<gop> ops;
+    std::vector<int2> items;
+    for (size_t q = 0; q < idx.size(); ++q) {
+      const auto &d = b->ops[(size_t)idx[q]];
+      gop o{};
+      o.v = view_of(d);
+      o.tiles_n = (d.N + kCfgs[grp.cfg].bn - 1) / kCfgs[grp.cfg].bn;
+      long long nt = tiles_of(d, grp.cfg);
+      for (long long t = 0; t < nt; ++t) items.push_back(make_int2((int)q, (int)t));
+      ops.push_back(o);
+      grp.flops += 2.0 * d.M * d.N * (double)d.K;
+    }
+    grp.nitems = (int)items.size();
+    int occ = occupancy(grp.cfg, grp.variant);
+    long long cap = (long long)b->ctx->num_cus * occ;
+    grp.grid = (int)(grp.nitems < cap ? grp.nitems : cap);
+    if (grp.grid < 1) grp.grid = 1;
+    bytes = (bytes + 255) / 256 * 256;
+    grp.ops_off = bytes;
+    bytes += ops.size() * sizeof(gop);
+    bytes = (bytes + 255) / 256 * 256;
+    grp.items_off = bytes;
+    bytes += items.size() * sizeof(int2);
+    bytes = (bytes + 255) / 256 * 256;
+    grp.counter_off = bytes;
+    bytes += 256;
+    b->groups.push_back(grp);
+    gops.push_back(std::move(ops));
+    gitems.push_back(std::move(items));
+  }
+  if (bytes > b->dev_bytes) {
+    if (b->dev) {
+      BBTS_HIP_CHECK(hipDeviceSynchronize());
+      BBTS_HIP_CHECK(hipFree(b->dev));
+      b->dev = nullptr;
+    }
+    BBTS_HIP_CHECK(hipMalloc(&b->dev, bytes));
+    b->dev_bytes = bytes;
+  }
+  if (bytes > 0) {
+    std::vector<char> host(bytes, 0);
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+      auto &grp = b->groups[gi];
+      std::memcpy(host.data() + grp.ops_off, gops[gi].data(), gops[gi].size() * sizeof(gop));
+      std::memcpy(host.data() + grp.items_off, gitems[gi].data(), gitems[gi].size() * sizeof(int2));
+    }
+    BBTS_HIP_CHECK(hipMemcpy(b->dev, host.data(), bytes, hipMemcpyHostToDevice));
+  }
+  b->planned = true;
+  return BBTS_OK;
+}
+
+int bbts_hip_batch_launch_group(bbts_batch *b, int g, bbts_stream_t stream) {
+  BBTS_REQUIRE(b && b->planned && g >= 0 && g < (int)b->groups.size(), BBTS_E_INVALID, "bad group index");
+  int rc = bind(b->ctx);
+  if (rc) return rc;
+  const auto &grp = b->groups[(size_t)g];
+  char *base = static_cast<char *>(b->dev);
+  grouped_fn fn = get_grouped(grp.cfg, grp.variant);
+  int bytes = smem_bytes(grp.cfg, grp.variant);
+  if ((rc = ensure_smem((const void *)fn, bytes))) return rc;
+  hipLaunchKernelGGL(fn, dim3((unsigned)grp.grid), dim3(256), (unsigned)bytes, (hipStream_t)stream,
+                     reinterpret_cast<const gop *>(base + grp.ops_off),
+                     reinterpret_cast<const int2 *>(base + grp.items_off), grp.nitems,
+                     reinterpret_cast<unsigned *>(base + grp.counter_off));
+  BBTS_LAUNCH_CHECK();
+  return BBTS_OK;
+}
+
+int bbts_hip_batch_num_groups(bbts_batch *b) { return b ? (int)b->groups.size() : 0; }
+
+int bbts_hip_batch_group_info(bbts_batch *b, int g, int *cfg, int *nitems, int *grid, double *flops) {
+  BBTS_REQUIRE(b && g >= 0 && g < (int)b->groups.size(), BBTS_E_INVALID, "bad group index");
+  const auto &grp = b->groups[(size_t)g];
+  if (cfg) *cfg = grp.cfg;
+  if (nitems) *nitems = grp.nitems;
+  if (grid) *grid = grp.grid;
+  if (flops) *flops = grp.flops;
+  return BBTS_OK;
+}
+
+int bbts_hip_batch_launch(bbts_batch *b, bbts_stream_t stream) {
+  BBTS_REQUIRE(b, BBTS_E_INVALID, "null batch");
+  int rc = bind(b->ctx);
+  if (rc) return rc;
+  if (!b->planned && (rc = bbts_hip_batch_plan(b, BBTS_SCHED_HEURISTIC, nullptr))) return rc;
+  for (int g = 0; g < (int)b->groups.size(); ++g)
+    if ((rc = bbts_hip_batch_launch_group(b, g, stream))) return rc;
+  return BBTS_OK;
+}
+
+int bbts_hip_batch_capture(bbts_batch *b, bbts_stream_t stream) {
+  BBTS_REQUIRE(b, BBTS_E_INVALID, "null batch");
+  int rc = bind(b->ctx);
+  if (rc) return rc;
+  if (!b->planned && (rc = bbts_hip_batch_plan(b, BBTS_SCHED_HEURISTIC, nullptr))) return rc;
+  drop_graph(b);
+  // kernel attributes must be set outside capture
+  for (auto &grp : b->groups)
+    if ((rc = ensure_smem((const void *)get_grouped(grp.cfg, grp.variant), smem_bytes(grp.cfg, grp.variant))))
+      return rc;
+  hipStream_t s = (hipStream_t)stream;
+  hipStream_t cap = s;
+  hipStream_t own = nullptr;
+  if (s == nullptr) {
+    BBTS_HIP_CHECK(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+    cap = own;
+  }
+  BBTS_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+  rc = bbts_hip_batch_launch(b, cap);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(cap, &graph);
+  if (own) (void)hipStreamDestroy(own);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) {
+    set_error("hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    return BBTS_E_HIP;
+  }
+  b->graph = graph;
+  BBTS_HIP_CHECK(hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0));
+  return BBTS_OK;
+}
+
+int bbts_hip_batch_replay(bbts_batch *b, bbts_stream_t stream) {
+  BBTS_REQUIRE(b, BBTS_E_INVALID, "null batch");
+  int rc = bind(b->ctx);
+  if (rc) return rc;
+  BBTS_REQUIRE(b->exec != nullptr, BBTS_E_INVALID, "batch has no captured graph");
+  BBTS_HIP_CHECK(hipGraphLaunch(b->exec, (hipStream_t)stream));
+  return BBTS_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Autotuner
+// ===========================================================================
+extern "C" {
+
+int bbts_hip_tune_lookup(bbts_hip_ctx *ctx, int M, int N, int K, int dtype, int precision, int *cfg, float *us) {
+  BBTS_REQUIRE(ctx && cfg, BBTS_E_INVALID, "null argument");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  auto it = ctx->tune_table.find(tune_key{M, N, K, dtype, dtype == 1 ? 0 : precision});
+  if (it == ctx->tune_table.end()) {
+    *cfg = -1;
+    if (us) *us = 0.0f;
+    return BBTS_OK;
+  }
+  *cfg = it->second.cfg;
+  if (us) *us = it->second.us;
+  return BBTS_OK;
+}
+
+int bbts_hip_tune_batch(bbts_batch *b, int reps, float *times_us) {
+  BBTS_REQUIRE(b && reps >= 1, BBTS_E_INVALID, "bad tune arguments");
+  int rc = bind(b->ctx);
+  if (rc) return rc;
+  const size_t n = b->ops.size();
+  std::map<tune_key, std::vector<size_t>> shapes;
+  for (size_t i = 0; i < n; ++i) {
+    const auto &d = b->ops[i];
+    shapes[tune_key{d.M, d.N, d.K, d.dtype, d.dtype == 1 ? 0 : b->prec[i]}].push_back(i);
+  }
+  if (times_us)
+    for (size_t i = 0; i < n * kNumCfgs; ++i) times_us[i] = -1.0f;
+  hipStream_t s;
+  BBTS_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  hipEvent_t e0, e1;
+  BBTS_HIP_CHECK(hipEventCreate(&e0));
+  BBTS_HIP_CHECK(hipEventCreate(&e1));
+  bbts_batch tmp;
+  tmp.ctx = b->ctx;
+  for (auto &kv : shapes) {
+    const bbts_gemm_desc_t &d = b->ops[kv.second[0]];
+    const int prec = b->prec[kv.second[0]];
+    if (d.M == 0 || d.N == 0) continue;
+    int variant;
+    if ((rc = variant_of(d.dtype, prec, &variant))) break;
+    int best = -1;
+    float best_us = 0.0f;
+    for (int c = 0; c < kNumCfgs; ++c) {
+      long long tiles = tiles_of(d, c);
+      long long target = 4LL * b->ctx->num_cus * occupancy(c, variant);
+      long long R = (target + tiles - 1) / tiles;
+      if (R < 1) R = 1;
+      if (R > 256) R = 256;
+      tmp.ops.assign((size_t)R, d);
+      tmp.prec.assign((size_t)R, prec);
+      tmp.planned = false;
+      if ((rc = bbts_hip_batch_plan(&tmp, BBTS_SCHED_FORCED_CFG0 + c, nullptr))) break;
+      if ((rc = bbts_hip_batch_launch(&tmp, s))) break;  // warm-up
+      hipEventRecord(e0, s);
+      for (int r = 0; r < reps && rc == BBTS_OK; ++r) rc = bbts_hip_batch_launch(&tmp, s);
+      hipEventRecord(e1, s);
+      if (rc) break;
+      hipError_t e = hipEventSynchronize(e1);
+      if (e != hipSuccess) {
+        set_error("tuning launch failed: %s", hipGetErrorString(e));
+        rc = BBTS_E_HIP;
+        break;
+      }
+      float ms = 0.0f;
+      hipEventElapsedTime(&ms, e0, e1);
+      float us = ms * 1000.0f / (float)(reps * R);
+      if (times_us)
+        for (size_t i : kv.second) times_us[i * kNumCfgs + (size_t)c] = us;
+      if (best < 0 || us < best_us) {
+        best = c;
+        best_us = us;
+      }
+    }
+    if (rc) break;
+    std::lock_guard<std::mutex> g(b->ctx->mu);
+    b->ctx->tune_table[kv.first] = tune_val{best, best_us};
+  }
+  (void)hipStreamSynchronize(s);
+  if (tmp.dev) (void)hipFree(tmp.dev);
+  tmp.dev = nullptr;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipStreamDestroy(s);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,572 @@
+// HBM-bound block kernels: add, k-way reduce, broadcast copy, ReLU, seeded
+// generators, BF16 conversion, mode permutation and slice sums.
+//
+// All of them stream: 16-byte loads/stores per lane, 4 independent vectors in
+// flight per lane, 256-lane workgroups, grids sized to the data.  None does
+// arithmetic beyond the reference's own (IEEE adds in the reference's order),
+// so add / reduce / ReLU / broadcast are bit-exact with the CPU ud functions.
+#include "common.h"
+
+namespace bbts_hip {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kVecPerThread = 4;
+
+__device__ inline void write_meta(uint32_t *dst, uint32_t w0, uint32_t w1) {
+  if (dst != nullptr && blockIdx.x == 0 && threadIdx.x == 0) {
+    dst[0] = w0;
+    dst[1] = w1;
+  }
+}
+
+// out = a + b (dense_matrix_add.cc:154-159; dense_matrix_gpu_add.cu:47-55)
+__global__ __launch_bounds__(kThreads) void add_vec_kernel(const float4 *__restrict__ a, const float4 *__restrict__ b,
+                                                           float4 *out, size_t n4, uint32_t *meta_dst, uint32_t w0,
+                                                           uint32_t w1) {
+  write_meta(meta_dst, w0, w1);
+  size_t base = (size_t)blockIdx.x * (kThreads * kVecPerThread) + threadIdx.x;
+  float4 va[kVecPerThread], vb[kVecPerThread];
+#pragma unroll
+  for (int j = 0; j < kVecPerThread; ++j) {
+    size_t i = base + (size_t)j * kThreads;
+    if (i < n4) {
+      va[j] = a[i];
+      vb[j] = b[i];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kVecPerThread; ++j) {
+    size_t i = base + (size_t)j * kThreads;
+    if (i < n4) {
+      float4 r;
+      r.x = va[j].x + vb[j].x;
+      r.y = va[j].y + vb[j].y;
+      r.z = va[j].z + vb[j].z;
+      r.w = va[j].w + vb[j].w;
+      out[i] = r;
+    }
+  }
+}
+
+__global__ void add_scalar_kernel(const float *a, const float *b, float *out, size_t start, size_t n,
+                                  uint32_t *meta_dst, uint32_t w0, uint32_t w1) {
+  write_meta(meta_dst, w0, w1);
+  size_t i = start + (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a[i] + b[i];
+}
+
+// k-way ordered left fold: out = ((in0 + in1) + in2) + ... (local_reduce_op.cc:23-91).
+constexpr int kMaxFold = 32;
+struct fold_args {
+  const float *in[kMaxFold];
+};
+
+template <bool VEC>
+__global__ __launch_bounds__(kThreads) void fold_kernel(fold_args args, int k, float *out, size_t n,
+                                                        uint32_t *meta_dst, uint32_t w0, uint32_t w1) {
+  write_meta(meta_dst, w0, w1);
+  if (VEC) {
+    // two float4 columns per lane; loads are issued four inputs at a time so
+    // eight vectors are in flight, and the adds stay in input order
+    size_t n4 = n / 4;
+    size_t i0 = (size_t)blockIdx.x * (kThreads * 2) + threadIdx.x;
+    size_t i1 = i0 + kThreads;
+    bool ok0 = i0 < n4, ok1 = i1 < n4;
+    const float4 *p0 = reinterpret_cast<const float4 *>(args.in[0]);
+    float4 a0 = ok0 ? p0[i0] : float4{0, 0, 0, 0};
+    float4 a1 = ok1 ? p0[i1] : float4{0, 0, 0, 0};
+    int q = 1;
+    for (; q + 3 < k; q += 4) {
+      float4 v0[4], v1[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float4 *pj = reinterpret_cast<const float4 *>(args.in[q + j]);
+        if (ok0) v0[j] = pj[i0];
+        if (ok1) v1[j] = pj[i1];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        a0.x = a0.x + v0[j].x; a0.y = a0.y + v0[j].y; a0.z = a0.z + v0[j].z; a0.w = a0.w + v0[j].w;
+        a1.x = a1.x + v1[j].x; a1.y = a1.y + v1[j].y; a1.z = a1.z + v1[j].z; a1.w = a1.w + v1[j].w;
+      }
+    }
+    for (; q < k; ++q) {
+      const float4 *pj = reinterpret_cast<const float4 *>(args.in[q]);
+      float4 v0 = ok0 ? pj[i0] : float4{0, 0, 0, 0};
+      float4 v1 = ok1 ? pj[i1] : float4{0, 0, 0, 0};
+      a0.x = a0.x + v0.x; a0.y = a0.y + v0.y; a0.z = a0.z + v0.z; a0.w = a0.w + v0.w;
+      a1.x = a1.x + v1.x; a1.y = a1.y + v1.y; a1.z = a1.z + v1.z; a1.w = a1.w + v1.w;
+    }
+    if (ok0) reinterpret_cast<float4 *>(out)[i0] = a0;
+    if (ok1) reinterpret_cast<float4 *>(out)[i1] = a1;
+  } else {
+    size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) {
+      float acc = args.in[0][i];
+      for (int q = 1; q < k; ++q) acc = acc + args.in[q][i];
+      out[i] = acc;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void relu_kernel(const float *in, float *out, size_t n, uint32_t *meta_dst,
+                                                        uint32_t w0, uint32_t w1, int vec) {
+  write_meta(meta_dst, w0, w1);
+  if (vec) {
+    size_t n4 = n / 4;
+    size_t base = (size_t)blockIdx.x * (kThreads * kVecPerThread) + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < kVecPerThread; ++j) {
+      size_t i = base + (size_t)j * kThreads;
+      if (i < n4) {
+        float4 v = reinterpret_cast<const float4 *>(in)[i];
+        v.x = v.x > 0.0f ? v.x : 0.0f;
+        v.y = v.y > 0.0f ? v.y : 0.0f;
+        v.z = v.z > 0.0f ? v.z : 0.0f;
+        v.w = v.w > 0.0f ? v.w : 0.0f;
+        reinterpret_cast<float4 *>(out)[i] = v;
+      }
+    }
+  } else {
+    size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) out[i] = in[i] > 0.0f ? in[i] : 0.0f;
+  }
+}
+
+// Replicate `bytes` from src to n destinations: every 16-byte chunk of the
+// source is read once and written n times.  head = bytes before the first
+// 16-byte boundary (identical misalignment for src and all dsts).
+constexpr int kMaxDst = 64;
+struct dst_args {
+  uint8_t *d[kMaxDst];
+};
+
+__global__ __launch_bounds__(kThreads) void copy_n_kernel(const uint8_t *src, size_t bytes, size_t head, dst_args dst,
+                                                          int n) {
+  size_t body = (bytes - head) / 16;
+  size_t tail_start = head + body * 16;
+  size_t gid = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * kThreads;
+  const uint4 *s4 = reinterpret_cast<const uint4 *>(src + head);
+  for (size_t i = gid; i < body; i += stride) {
+    uint4 v = s4[i];
+    for (int d = 0; d < n; ++d) reinterpret_cast<uint4 *>(dst.d[d] + head)[i] = v;
+  }
+  if (gid < head) {
+    uint8_t v = src[gid];
+    for (int d = 0; d < n; ++d) dst.d[d][gid] = v;
+  }
+  if (gid < bytes - tail_start) {
+    uint8_t v = src[tail_start + gid];
+    for (int d = 0; d < n; ++d) dst.d[d][tail_start + gid] = v;
+  }
+}
+
+__global__ void copy_bytes_kernel(const uint8_t *src, size_t bytes, dst_args dst, int n) {
+  size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = gid; i < bytes; i += stride) {
+    uint8_t v = src[i];
+    for (int d = 0; d < n; ++d) dst.d[d][i] = v;
+  }
+}
+
+// splitmix64 at counter i (S. Vigna's published generator, evaluated counter-wise)
+__device__ inline uint64_t splitmix(uint64_t seed, uint64_t i) {
+  uint64_t z = seed + (i + 1ull) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(kThreads) void uniform_kernel(float *out, size_t n, uint64_t seed, uint64_t offset,
+                                                           float left, float right, uint32_t *meta_dst, uint32_t w0,
+                                                           uint32_t w1) {
+  write_meta(meta_dst, w0, w1);
+  size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  uint64_t x = splitmix(seed, offset + i);
+  float u = __fmul_rn((float)(x >> 40), 1.0f / 16777216.0f);
+  float w = __fsub_rn(right, left);
+  out[i] = __fadd_rn(left, __fmul_rn(w, u));
+}
+
+__global__ __launch_bounds__(kThreads) void normal_kernel(float *out, size_t n, uint64_t seed, uint64_t offset,
+                                                          double mean, double stdev) {
+  size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  uint64_t j = offset + i;
+  uint64_t x1 = splitmix(seed, 2ull * j);
+  uint64_t x2 = splitmix(seed, 2ull * j + 1ull);
+  double u1 = __dmul_rn((double)(x1 >> 11) + 1.0, 1.0 / 9007199254740992.0);
+  double u2 = __dmul_rn((double)(x2 >> 11), 1.0 / 9007199254740992.0);
+  double r = sqrt(__dmul_rn(-2.0, log(u1)));
+  double z = __dmul_rn(r, cos(__dmul_rn(6.283185307179586476925286766559, u2)));
+  out[i] = (float)__dadd_rn(mean, __dmul_rn(stdev, z));
+}
+
+__device__ inline uint16_t f32_to_bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__global__ __launch_bounds__(kThreads) void to_bf16_kernel(const float *in, uint16_t *out, size_t n) {
+  size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) out[i] = f32_to_bf16_rne(in[i]);
+}
+
+// Column-major mode permutation (rank <= 4): out mode r = in mode perm[r].
+// One lane per output element; reads are gathered, writes coalesced.
+struct perm_args {
+  int64_t out_dims[4];
+  int64_t in_stride_of_out_mode[4];
+};
+
+__global__ __launch_bounds__(kThreads) void permute_kernel(const float *in, float *out, size_t n, int rank,
+                                                           perm_args p) {
+  size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  size_t rem = i;
+  int64_t off = 0;
+  for (int r = 0; r < rank; ++r) {
+    int64_t d = p.out_dims[r];
+    int64_t idx = (int64_t)(rem % (size_t)d);
+    rem /= (size_t)d;
+    off += idx * p.in_stride_of_out_mode[r];
+  }
+  out[i] = in[off];
+}
+
+// out[e] = in[e] + in[slice+e] + ... (left fold over nslices slices; reduce_t, contraction.h:379-412)
+__global__ __launch_bounds__(kThreads) void sum_slices_kernel(const float *in, int64_t slice, int64_t nslices,
+                                                              float *out) {
+  int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (e >= slice) return;
+  float acc = in[e];
+  for (int64_t s = 1; s < nslices; ++s) acc = acc + in[s * slice + e];
+  out[e] = acc;
+}
+
+inline unsigned grid_for(size_t items, size_t per_block) {
+  size_t g = (items + per_block - 1) / per_block;
+  return (unsigned)(g == 0 ? 1 : g);
+}
+
+inline bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+}  // namespace
+
+int launch_add(const float *a, const float *b, float *out, size_t n, uint32_t *meta_dst, uint32_t w0, uint32_t w1,
+               hipStream_t s) {
+  if (n == 0) {
+    if (meta_dst) {
+      hipLaunchKernelGGL(add_scalar_kernel, dim3(1), dim3(64), 0, s, a, b, out, (size_t)0, (size_t)0, meta_dst, w0,
+                         w1);
+      BBTS_LAUNCH_CHECK();
+    }
+    return BBTS_OK;
+  }
+  size_t done = 0;
+  if (aligned16(a) && aligned16(b) && aligned16(out) && n >= 4) {
+    size_t n4 = n / 4;
+    hipLaunchKernelGGL(add_vec_kernel, dim3(grid_for(n4, kThreads * kVecPerThread)), dim3(kThreads), 0, s,
+                       reinterpret_cast<const float4 *>(a), reinterpret_cast<const float4 *>(b),
+                       reinterpret_cast<float4 *>(out), n4, meta_dst, w0, w1);
+    BBTS_LAUNCH_CHECK();
+    done = n4 * 4;
+    meta_dst = nullptr;
+  }
+  if (done < n) {
+    hipLaunchKernelGGL(add_scalar_kernel, dim3(grid_for(n - done, kThreads)), dim3(kThreads), 0, s, a, b, out, done, n,
+                       meta_dst, w0, w1);
+    BBTS_LAUNCH_CHECK();
+  }
+  return BBTS_OK;
+}
+
+int launch_reduce(const float *const *in, int k, float *out, size_t n, uint32_t *meta_dst, uint32_t w0, uint32_t w1,
+                  hipStream_t s) {
+  BBTS_REQUIRE(k >= 1, BBTS_E_INVALID, "reduce needs at least one input (k=%d)", k);
+  // Fold in chunks of up to kMaxFold inputs; later chunks start from `out`,
+  // which keeps the exact left-fold order across chunks.
+  int pos = 0;
+  bool first = true;
+  while (first || pos < k) {
+    fold_args args;
+    int cnt = 0;
+    if (!first) args.in[cnt++] = out;
+    while (cnt < kMaxFold && pos < k) args.in[cnt++] = in[pos++];
+    bool vec = (n % 4 == 0) && aligned16(out);
+    for (int q = 0; q < cnt; ++q) vec = vec && aligned16(args.in[q]);
+    if (vec) {
+      hipLaunchKernelGGL(fold_kernel<true>, dim3(grid_for(n / 4, kThreads * 2)), dim3(kThreads), 0, s, args, cnt, out,
+                         n, meta_dst, w0, w1);
+    } else {
+      hipLaunchKernelGGL(fold_kernel<false>, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, s, args, cnt, out, n,
+                         meta_dst, w0, w1);
+    }
+    BBTS_LAUNCH_CHECK();
+    meta_dst = nullptr;
+    first = false;
+  }
+  return BBTS_OK;
+}
+
+int launch_sum_slices(const float *in, int64_t slice, int64_t nslices, float *out, hipStream_t s) {
+  if (slice <= 0) return BBTS_OK;
+  hipLaunchKernelGGL(sum_slices_kernel, dim3(grid_for((size_t)slice, kThreads)), dim3(kThreads), 0, s, in, slice,
+                     nslices, out);
+  BBTS_LAUNCH_CHECK();
+  return BBTS_OK;
+}
+
+int launch_permute(const float *in, float *out, int rank, const int64_t *dims, const int *perm, hipStream_t s) {
+  BBTS_REQUIRE(rank >= 0 && rank <= 4, BBTS_E_INVALID, "rank %d > 4", rank);
+  perm_args p{};
+  int64_t in_stride[4];
+  int64_t st = 1;
+  size_t n = 1;
+  for (int r = 0; r < rank; ++r) {
+    in_stride[r] = st;
+    st *= dims[r];
+    n *= (size_t)dims[r];
+  }
+  for (int r = 0; r < rank; ++r) {
+    p.out_dims[r] = dims[perm[r]];
+    p.in_stride_of_out_mode[r] = in_stride[perm[r]];
+  }
+  if (n == 0) return BBTS_OK;
+  hipLaunchKernelGGL(permute_kernel, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, s, in, out, n, rank, p);
+  BBTS_LAUNCH_CHECK();
+  return BBTS_OK;
+}
+
+}  // namespace bbts_hip
+
+using namespace bbts_hip;
+
+namespace {
+
+int check_dense(bbts_hip_ctx *ctx, const bbts_tensor_t *t, block_rec *r, dense_m_t *m) {
+  int rc = lookup(ctx, t, r);
+  if (rc) return rc;
+  *m = dense_meta(r->meta);
+  size_t need = bbts_hip_dense_size(m->num_rows, m->num_cols);
+  BBTS_REQUIRE(need <= r->bytes, BBTS_E_SHAPE, "dense block %ux%u needs %zu bytes, block has %zu", m->num_rows,
+               m->num_cols, need, r->bytes);
+  return BBTS_OK;
+}
+
+int set_out_dense(bbts_hip_ctx *ctx, bbts_tensor_t *out, const block_rec &r, uint32_t rows, uint32_t cols) {
+  BBTS_REQUIRE(bbts_hip_dense_size(rows, cols) <= r.bytes, BBTS_E_SHAPE,
+               "output block of %zu bytes cannot hold %ux%u", r.bytes, rows, cols);
+  tensor_meta_t m = r.meta;
+  set_dense_meta(m, rows, cols);
+  return update_meta(ctx, out, m);
+}
+
+}  // namespace
+
+extern "C" {
+
+int bbts_hip_add_f32(bbts_hip_ctx *ctx, const float *a, const float *b, float *out, size_t n, bbts_stream_t stream) {
+  int rc = bind(ctx);
+  if (rc) return rc;
+  BBTS_REQUIRE(n == 0 || (a && b && out), BBTS_E_INVALID, "null pointer");
+  return launch_add(a, b, out, n, nullptr, 0, 0, (hipStream_t)stream);
+}
+
+int bbts_hip_reduce_f32(bbts_hip_ctx *ctx, const float *const *in, int k, float *out, size_t n,
+                        bbts_stream_t stream) {
+  int rc = bind(ctx);
+  if (rc) return rc;
+  BBTS_REQUIRE(in && out && k >= 1, BBTS_E_INVALID, "bad reduce arguments");
+  for (int q = 0; q < k; ++q) BBTS_REQUIRE(in[q], BBTS_E_INVALID, "null input %d", q);
+  if (n == 0) return BBTS_OK;
+  return launch_reduce(in, k, out, n, nullptr, 0, 0, (hipStream_t)stream);
+}
+
+int bbts_hip_relu_f32(bbts_hip_ctx *ctx, const float *in, float *out, size_t n, bbts_stream_t stream) {
+  int rc = bind(ctx);
+  if (rc) return rc;
+  if (n == 0) return BBTS_OK;
+  BBTS_REQUIRE(in && out, BBTS_E_INVALID, "null pointer");
+  int vec = (n % 4 == 0) && aligned16(in) && aligned16(out);
+  unsigned grid = vec ? grid_for(n / 4, kThreads * kVecPerThread) : grid_for(n, kThreads);
+  hipLaunchKernelGGL(relu_kernel, dim3(grid), dim3(kThreads), 0, (hipStream_t)stream, in, out, n, nullptr, 0u, 0u,
+                     vec);
+  BBTS_LAUNCH_CHECK();
+  return BBTS_OK;
+}
+
+int bbts_hip_copy_n(bbts_hip_ctx *ctx, const void *src, size_t bytes, void *const *dst, int n, bbts_stream_t stream) {
+  int rc = bind(ctx);
+  if (rc) return rc;
+  BBTS_REQUIRE(src && dst && n >= 0, BBTS_E_INVALID, "bad copy arguments");
+  if (bytes == 0 || n == 0) return BBTS_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const uint8_t *s8 = static_cast<const uint8_t *>(src);
+  size_t mis = reinterpret_cast<uintptr_t>(s8) & 15u;
+  for (int base = 0; base < n; base += kMaxDst) {
+    int cnt = n - base < kMaxDst ? n - base : kMaxDst;
+    dst_args d{};
+    bool same = true;
+    for (int q = 0; q < cnt; ++q) {
+      BBTS_REQUIRE(dst[base + q], BBTS_E_INVALID, "null destination %d", base + q);
+      d.d[q] = static_cast<uint8_t *>(dst[base + q]);
+      same = same && ((reinterpret_cast<uintptr_t>(d.d[q]) & 15u) == mis);
+    }
+    if (same) {
+      size_t head = mis ? 16 - mis : 0;
+      if (head > bytes) head = bytes;
+      size_t body = (bytes - head) / 16;
+      unsigned grid = grid_for(body > 0 ? body : 1, kThreads);
+      unsigned cap = 8u * 256u;
+      if (grid > cap) grid = cap;
+      hipLaunchKernelGGL(copy_n_kernel, dim3(grid), dim3(kThreads), 0, s, s8, bytes, head, d, cnt);
+    } else {
+      unsigned grid = grid_for(bytes, kThreads);
+      if (grid > 4096u) grid = 4096u;
+      hipLaunchKernelGGL(copy_bytes_kernel, dim3(grid), dim3(kThreads), 0, s, s8, bytes, d, cnt);
+    }
+    BBTS_LAUNCH_CHECK();
+  }
+  return BBTS_OK;
+}
+
+int bbts_hip_gen_uniform(bbts_hip_ctx *ctx, float *out, size_t n, uint64_t seed, uint64_t offset, float left,
+                         float right, bbts_stream_t stream) {
+  int rc = bind(ctx);
+  if (rc) return rc;
+  if (n == 0) return BBTS_OK;
+  BBTS_REQUIRE(out, BBTS_E_INVALID, "null output");
+  hipLaunchKernelGGL(uniform_kernel, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, (hipStream_t)stream, out, n, seed,
+                     offset, left, right, nullptr, 0u, 0u);
+  BBTS_LAUNCH_CHECK();
+  return BBTS_OK;
+}
+
+int bbts_hip_gen_normal(bbts_hip_ctx *ctx, float *out, size_t n, uint64_t seed, uint64_t offset, double mean,
+                        double stdev, bbts_stream_t stream) {
+  int rc = bind(ctx);
+  if (rc) return rc;
+  if (n == 0) return BBTS_OK;
+  BBTS_REQUIRE(out, BBTS_E_INVALID, "null output");
+  hipLaunchKernelGGL(normal_kernel, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, (hipStream_t)stream, out, n, seed,
+                     offset, mean, stdev);
+  BBTS_LAUNCH_CHECK();
+  return BBTS_OK;
+}
+
+int bbts_hip_f32_to_bf16(bbts_hip_ctx *ctx, const float *in, uint16_t *out, size_t n, bbts_stream_t stream) {
+  int rc = bind(ctx);
+  if (rc) return rc;
+  if (n == 0) return BBTS_OK;
+  BBTS_REQUIRE(in && out, BBTS_E_INVALID, "null pointer");
+  hipLaunchKernelGGL(to_bf16_kernel, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, (hipStream_t)stream, in, out, n);
+  BBTS_LAUNCH_CHECK();
+  return BBTS_OK;
+}
+
+// ---- block-level ud functions ----------------------------------------------
+
+int bbts_hip_dense_add(bbts_hip_ctx *ctx, const bbts_tensor_t *a, const bbts_tensor_t *b, bbts_tensor_t *out,
+                       bbts_stream_t stream) {
+  int rc = bind(ctx);
+  if (rc) return rc;
+  block_rec ra, rb, ro;
+  dense_m_t ma, mb, mo;
+  if ((rc = check_dense(ctx, a, &ra, &ma)) || (rc = check_dense(ctx, b, &rb, &mb))) return rc;
+  if ((rc = lookup(ctx, out, &ro))) return rc;
+  (void)mo;
+  // dense_matrix_add.cc:68-73 asserts equal shapes (debug builds); checked always here
+  BBTS_REQUIRE(ma.num_rows == mb.num_rows && ma.num_cols == mb.num_cols, BBTS_E_SHAPE,
+               "matrix_add shapes %ux%u and %ux%u differ", ma.num_rows, ma.num_cols, mb.num_rows, mb.num_cols);
+  if ((rc = set_out_dense(ctx, out, ro, ma.num_rows, ma.num_cols))) return rc;
+  size_t n = (size_t)ma.num_rows * ma.num_cols;
+  return launch_add(payload(a), payload(b), payload(out), n, meta_words(out), ma.num_rows, ma.num_cols,
+                    (hipStream_t)stream);
+}
+
+int bbts_hip_dense_reduce(bbts_hip_ctx *ctx, const bbts_tensor_t *const *in, int k, bbts_tensor_t *out,
+                          bbts_stream_t stream) {
+  int rc = bind(ctx);
+  if (rc) return rc;
+  BBTS_REQUIRE(in && k >= 1, BBTS_E_INVALID, "reduce needs k >= 1 inputs");
+  block_rec r0, ro;
+  dense_m_t m0;
+  if ((rc = check_dense(ctx, in[0], &r0, &m0))) return rc;
+  std::vector<const float *> ptrs((size_t)k);
+  ptrs[0] = payload(in[0]);
+  for (int q = 1; q < k; ++q) {
+    block_rec rq;
+    dense_m_t mq;
+    if ((rc = check_dense(ctx, in[q], &rq, &mq))) return rc;
+    BBTS_REQUIRE(mq.num_rows == m0.num_rows && mq.num_cols == m0.num_cols, BBTS_E_SHAPE,
+                 "reduce input %d is %ux%u, input 0 is %ux%u", q, mq.num_rows, mq.num_cols, m0.num_rows, m0.num_cols);
+    ptrs[(size_t)q] = payload(in[q]);
+  }
+  if ((rc = lookup(ctx, out, &ro))) return rc;
+  if ((rc = set_out_dense(ctx, out, ro, m0.num_rows, m0.num_cols))) return rc;
+  size_t n = (size_t)m0.num_rows * m0.num_cols;
+  return launch_reduce(ptrs.data(), k, payload(out), n, meta_words(out), m0.num_rows, m0.num_cols,
+                       (hipStream_t)stream);
+}
+
+int bbts_hip_broadcast(bbts_hip_ctx *ctx, const bbts_tensor_t *src, bbts_tensor_t *const *dst, int n,
+                       bbts_stream_t stream) {
+  int rc = bind(ctx);
+  if (rc) return rc;
+  BBTS_REQUIRE(dst && n >= 0, BBTS_E_INVALID, "bad broadcast arguments");
+  block_rec rs;
+  if ((rc = lookup(ctx, src, &rs))) return rc;
+  std::vector<void *> d((size_t)n);
+  for (int q = 0; q < n; ++q) {
+    block_rec rd;
+    if ((rc = lookup(ctx, dst[q], &rd))) return rc;
+    BBTS_REQUIRE(rd.bytes >= rs.bytes, BBTS_E_SHAPE, "destination %d holds %zu bytes, tensor is %zu", q, rd.bytes,
+                 rs.bytes);
+    d[(size_t)q] = dst[q];
+  }
+  if ((rc = bbts_hip_copy_n(ctx, src, rs.bytes, d.data(), n, stream))) return rc;
+  for (int q = 0; q < n; ++q)
+    if ((rc = update_meta(ctx, dst[q], rs.meta))) return rc;
+  return BBTS_OK;
+}
+
+int bbts_hip_dense_relu(bbts_hip_ctx *ctx, const bbts_tensor_t *a, bbts_tensor_t *out, bbts_stream_t stream) {
+  int rc = bind(ctx);
+  if (rc) return rc;
+  block_rec ra, ro;
+  dense_m_t ma;
+  if ((rc = check_dense(ctx, a, &ra, &ma))) return rc;
+  if ((rc = lookup(ctx, out, &ro))) return rc;
+  if ((rc = set_out_dense(ctx, out, ro, ma.num_rows, ma.num_cols))) return rc;
+  size_t n = (size_t)ma.num_rows * ma.num_cols;
+  int vec = (n % 4 == 0);
+  unsigned grid = vec ? grid_for(n / 4, kThreads * kVecPerThread) : grid_for(n, kThreads);
+  hipLaunchKernelGGL(relu_kernel, dim3(grid), dim3(kThreads), 0, (hipStream_t)stream, payload(a), payload(out), n,
+                     meta_words(out), ma.num_rows, ma.num_cols, vec);
+  BBTS_LAUNCH_CHECK();
+  return BBTS_OK;
+}
+
+int bbts_hip_dense_uniform(bbts_hip_ctx *ctx, bbts_tensor_t *out, uint32_t rows, uint32_t cols, float left,
+                           float right, uint64_t seed, bbts_stream_t stream) {
+  int rc = bind(ctx);
+  if (rc) return rc;
+  block_rec ro;
+  if ((rc = lookup(ctx, out, &ro))) return rc;
+  if ((rc = set_out_dense(ctx, out, ro, rows, cols))) return rc;
+  size_t n = (size_t)rows * cols;
+  hipLaunchKernelGGL(uniform_kernel, dim3(grid_for(n ? n : 1, kThreads)), dim3(kThreads), 0, (hipStream_t)stream,
+                     payload(out), n, seed, (uint64_t)0, left, right, meta_words(out), rows, cols);
+  BBTS_LAUNCH_CHECK();
+  return BBTS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,283 @@
+"""Python view of the C ABI: a device context, HBM blocks with the reference's
+260-byte headers, and the block ud functions.
+
+Blocks are allocated by libbbts_hip.so's pool; raw-pointer entry points take
+torch device tensors (torch is only used for device memory and streams here).
+"""
+import ctypes
+import struct
+
+import numpy as np
+
+from . import META_BYTES, PREC_BF16X3, call, lib
+
+DENSE_FMT = 0      # "dense" is the first registered format (src/tensor/tensor_factory.cc:57)
+CUTENSOR_FMT = 1   # "cutensor" is registered next by the library plugin (libraries/cutensor.cc:16-24)
+
+
+def dense_header(rows, cols, fmt_id=DENSE_FMT):
+    """tensor_meta_t bytes: int32 fmt_id, then dense m_t {u32 rows, u32 cols}, rest zero."""
+    h = bytearray(META_BYTES)
+    struct.pack_into("<iII", h, 0, fmt_id, rows, cols)
+    return bytes(h)
+
+
+def cu_header(dims, fmt_id=CUTENSOR_FMT):
+    """tensor_meta_t bytes with cu_shape_t {u32 rank; (pad); i64 dims[4]} at the blob."""
+    h = bytearray(META_BYTES)
+    struct.pack_into("<iI", h, 0, fmt_id, len(dims))
+    for r, d in enumerate(dims):
+        struct.pack_into("<q", h, 4 + 8 + 8 * r, int(d))
+    return bytes(h)
+
+
+def dense_size(rows, cols):
+    return META_BYTES + 4 * rows * cols
+
+
+def cu_size(dims):
+    n = 1
+    for d in dims:
+        n *= int(d)
+    return META_BYTES + 4 * n
+
+
+class Block:
+    """A block in HBM: `ptr` is the device address of its 260-byte header."""
+
+    def __init__(self, ctx, ptr, nbytes):
+        self.ctx = ctx
+        self.ptr = ptr
+        self.nbytes = nbytes
+
+    @property
+    def data_ptr(self):
+        return self.ptr + META_BYTES
+
+    def meta(self):
+        buf = ctypes.create_string_buffer(META_BYTES)
+        nb = ctypes.c_size_t()
+        call("bbts_hip_block_meta", self.ctx.h, self.ptr, buf, ctypes.byref(nb))
+        return bytes(buf.raw)
+
+    def dense_shape(self):
+        _, r, c = struct.unpack_from("<iII", self.meta(), 0)
+        return r, c
+
+    def cu_shape(self):
+        m = self.meta()
+        rank = struct.unpack_from("<I", m, 4)[0]
+        return tuple(struct.unpack_from("<q", m, 12 + 8 * r)[0] for r in range(rank))
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float32)
+        call("bbts_hip_block_upload", self.ctx.h, self.ptr, a.ctypes.data, a.nbytes, None)
+
+    def download_bytes(self, nbytes=None):
+        n = self.nbytes if nbytes is None else nbytes
+        out = np.empty(n, dtype=np.uint8)
+        call("bbts_hip_block_download", self.ctx.h, self.ptr, out.ctypes.data, n, None)
+        return out
+
+    def download_payload(self, count):
+        raw = self.download_bytes(META_BYTES + 4 * count)
+        return raw[META_BYTES:].view(np.float32).copy()
+
+    def free(self):
+        if self.ptr:
+            call("bbts_hip_block_free", self.ctx.h, self.ptr)
+            self.ptr = None
+
+
+class Context:
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        call("bbts_hip_init", device, ctypes.byref(h))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            call("bbts_hip_shutdown", self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def info(self):
+        cus, xcds, hbm = ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()
+        call("bbts_hip_device_info", self.h, ctypes.byref(cus), ctypes.byref(xcds), ctypes.byref(hbm))
+        return {"cus": cus.value, "xcds": xcds.value, "hbm_bytes": hbm.value}
+
+    def sync(self, stream=None):
+        call("bbts_hip_sync", self.h, stream)
+
+    # ---- blocks
+    def alloc(self, header, nbytes):
+        p = ctypes.c_void_p()
+        call("bbts_hip_block_alloc", self.h, header, nbytes, ctypes.byref(p))
+        return Block(self, p.value, nbytes)
+
+    def dense(self, rows, cols, data=None, fmt_id=DENSE_FMT):
+        b = self.alloc(dense_header(rows, cols, fmt_id), dense_size(rows, cols))
+        if data is not None:
+            b.upload(np.asarray(data, dtype=np.float32).reshape(-1))
+        return b
+
+    def cutensor(self, dims, data=None, fmt_id=CUTENSOR_FMT):
+        b = self.alloc(cu_header(dims, fmt_id), cu_size(dims))
+        if data is not None:
+            b.upload(np.asarray(data, dtype=np.float32).reshape(-1))
+        return b
+
+    def pool_stats(self):
+        a, c = ctypes.c_size_t(), ctypes.c_size_t()
+        call("bbts_hip_pool_stats", self.h, ctypes.byref(a), ctypes.byref(c))
+        return a.value, c.value
+
+    # ---- ud functions on blocks
+    def matmul(self, a, b, out, trans_a=False, trans_b=False, precision=0, stream=None):
+        call("bbts_hip_dense_matmul", self.h, a.ptr, b.ptr, out.ptr, int(trans_a), int(trans_b), precision, stream)
+
+    def add(self, a, b, out, stream=None):
+        call("bbts_hip_dense_add", self.h, a.ptr, b.ptr, out.ptr, stream)
+
+    def reduce(self, ins, out, stream=None):
+        arr = (ctypes.c_void_p * len(ins))(*[x.ptr for x in ins])
+        call("bbts_hip_dense_reduce", self.h, arr, len(ins), out.ptr, stream)
+
+    def broadcast(self, src, dsts, stream=None):
+        arr = (ctypes.c_void_p * len(dsts))(*[x.ptr for x in dsts])
+        call("bbts_hip_broadcast", self.h, src.ptr, arr, len(dsts), stream)
+
+    def relu(self, a, out, stream=None):
+        call("bbts_hip_dense_relu", self.h, a.ptr, out.ptr, stream)
+
+    def uniform(self, out, rows, cols, left, right, seed, stream=None):
+        call("bbts_hip_dense_uniform", self.h, out.ptr, rows, cols, left, right, seed, stream)
+
+    def contraction(self, lhs, rhs, out, params, precision=0, stream=None):
+        p = np.ascontiguousarray(params, dtype=np.int32)
+        call("bbts_hip_contraction", self.h, lhs.ptr, rhs.ptr, out.ptr,
+             p.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(p), precision, stream)
+
+    # ---- raw entry points on torch device tensors
+    def gemm(self, A, B, C, M, N, K, lda, ldb, ldc, ta=False, tb=False, alpha=1.0, precision=0, stream=None):
+        call("bbts_hip_gemm", self.h, M, N, K, A.data_ptr(), lda, int(ta), B.data_ptr(), ldb, int(tb),
+             C.data_ptr(), ldc, alpha, precision, stream)
+
+    def residual(self, M, N, K, A, lda, a_bf16, B, ldb, b_bf16, C, ldc, out2, stream=None):
+        call("bbts_hip_residual_f64", self.h, M, N, K, A.data_ptr(), lda, int(a_bf16), B.data_ptr(), ldb,
+             int(b_bf16), C.data_ptr(), ldc, out2.data_ptr(), stream)
+
+    def gen_uniform(self, t, seed, left=-1.0, right=1.0, offset=0, stream=None):
+        call("bbts_hip_gen_uniform", self.h, t.data_ptr(), t.numel(), seed, offset, left, right, stream)
+
+    def gen_normal(self, t, seed, mean=0.0, std=1.0, offset=0, stream=None):
+        call("bbts_hip_gen_normal", self.h, t.data_ptr(), t.numel(), seed, offset, mean, std, stream)
+
+    def to_bf16(self, src, dst, stream=None):
+        call("bbts_hip_f32_to_bf16", self.h, src.data_ptr(), dst.data_ptr(), src.numel(), stream)
+
+
+class GemmDesc(ctypes.Structure):
+    _fields_ = [("M", ctypes.c_int32), ("N", ctypes.c_int32), ("K", ctypes.c_int32), ("dtype", ctypes.c_int32),
+                ("A", ctypes.c_void_p), ("B", ctypes.c_void_p), ("C", ctypes.c_void_p),
+                ("lda", ctypes.c_int64), ("ldb", ctypes.c_int64), ("ldc", ctypes.c_int64)]
+
+
+class Batch:
+    """Grouped stream of block GEMMs (bbts_hip_batch_*)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        call("bbts_hip_batch_create", ctx.h, ctypes.byref(h))
+        self.h = h
+        self.n = 0
+
+    def add(self, M, N, K, A, B, C, lda=None, ldb=None, ldc=None, dtype=0, precision=PREC_BF16X3):
+        d = GemmDesc(M, N, K, dtype, A, B, C, lda if lda is not None else K, ldb if ldb is not None else N,
+                     ldc if ldc is not None else N)
+        call("bbts_hip_batch_add_gemm", self.h, ctypes.byref(d), precision)
+        self.n += 1
+
+    def plan(self, policy, costmodel=None):
+        call("bbts_hip_batch_plan", self.h, policy, costmodel.h if costmodel is not None else None)
+
+    def launch(self, stream=None):
+        call("bbts_hip_batch_launch", self.h, stream)
+
+    def capture(self, stream=None):
+        call("bbts_hip_batch_capture", self.h, stream)
+
+    def replay(self, stream=None):
+        call("bbts_hip_batch_replay", self.h, stream)
+
+    def op_config(self, i):
+        c = ctypes.c_int()
+        call("bbts_hip_batch_op_config", self.h, i, ctypes.byref(c))
+        return c.value
+
+    def tune(self, reps=3):
+        nc = lib().bbts_hip_num_tile_configs()
+        t = np.zeros(self.n * nc, dtype=np.float32)
+        call("bbts_hip_tune_batch", self.h, reps, t.ctypes.data)
+        return t.reshape(self.n, nc)
+
+    def close(self):
+        if self.h:
+            call("bbts_hip_batch_destroy", self.h)
+            self.h = None
+
+
+class CostModel:
+    """The MLP cost model (trained and evaluated by the library's HIP kernels)."""
+
+    def __init__(self, ctx, hidden=64, seed=0):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        call("bbts_hip_costmodel_create", ctx.h, hidden, seed, ctypes.byref(h))
+        self.h = h
+
+    @staticmethod
+    def num_features():
+        return lib().bbts_hip_costmodel_num_features()
+
+    @staticmethod
+    def features(M, N, K, dtype, precision, cfg):
+        f = np.zeros(CostModel.num_features(), dtype=np.float32)
+        call("bbts_hip_costmodel_features", M, N, K, dtype, precision, cfg, f.ctypes.data)
+        return f
+
+    def num_params(self):
+        return lib().bbts_hip_costmodel_num_params(self.h)
+
+    def get_params(self):
+        p = np.zeros(self.num_params(), dtype=np.float32)
+        call("bbts_hip_costmodel_get_params", self.h, p.ctypes.data, len(p))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        call("bbts_hip_costmodel_set_params", self.h, p.ctypes.data, len(p))
+
+    def train(self, X, y, batch=256, lr=1e-3, epochs=1):
+        loss = ctypes.c_float()
+        call("bbts_hip_costmodel_train", self.h, X.data_ptr(), y.data_ptr(), X.shape[0], batch, lr, epochs,
+             ctypes.byref(loss), None)
+        return loss.value
+
+    def predict(self, X, y):
+        call("bbts_hip_costmodel_predict", self.h, X.data_ptr(), X.shape[0], y.data_ptr(), None)
+
+    def grad(self, X, y, grads, loss):
+        call("bbts_hip_costmodel_grad", self.h, X.data_ptr(), y.data_ptr(), X.shape[0], grads.data_ptr(),
+             loss.data_ptr(), None)
+
+    def close(self):
+        if self.h:
+            call("bbts_hip_costmodel_destroy", self.h)
+            self.h = None

@@ -1,0 +1,224 @@
+/*
+ * bbts_hip.h -- C ABI of the MI355X block-kernel library (libbbts_hip.so).
+ *
+ * The library replaces the per-node kernels of BarbaTOS (dcbdan/bbts): the
+ * dense ud functions (matrix_mult, matrix_add, uniform), the local reduce,
+ * the single-node meaning of broadcast, and the cutensor-library contraction.
+ * Plain C types only: device pointers, sizes, and an opaque hipStream_t
+ * passed as void*.  Every function returns 0 (BBTS_OK) or a bbts_status code;
+ * bbts_hip_last_error() gives the message of the calling thread's last failure.
+ *
+ * Blocks.  A block is one device allocation holding the reference's 260-byte
+ * header (int32 fmt_id + 256 meta bytes, src/tensor/tensor.h:20-34) followed
+ * by the payload (tensor.h:50-51).  A bbts_tensor_t* is the DEVICE address
+ * of that header; the payload is at +260.  The allocator places the header at
+ * base+252 of a 256-byte aligned allocation so the payload is 256-byte aligned.
+ * The library keeps a host copy of every block's header (the reference
+ * reads headers from host code: src/commands/command_runner.cc:110), so shape
+ * lookups never touch the device.
+ */
+#ifndef BBTS_HIP_H
+#define BBTS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bbts_hip_ctx bbts_hip_ctx;
+typedef struct bbts_tensor bbts_tensor_t;    /* device address of a block header */
+typedef struct bbts_batch bbts_batch;
+typedef struct bbts_costmodel bbts_costmodel;
+typedef void *bbts_stream_t;                 /* hipStream_t; NULL = the null stream */
+
+#define BBTS_META_BYTES 260                  /* sizeof(tensor_meta_t), src/tensor/tensor.h:20-34 */
+
+enum bbts_status {
+  BBTS_OK = 0,
+  BBTS_E_INVALID = 1,        /* bad argument (null pointer, negative size, bad params) */
+  BBTS_E_SHAPE = 2,          /* operand shapes do not match */
+  BBTS_E_UNKNOWN_BLOCK = 3,  /* pointer is not a block of this context */
+  BBTS_E_OOM = 4,            /* device allocation failed */
+  BBTS_E_HIP = 5,            /* a HIP runtime call failed */
+  BBTS_E_UNSUPPORTED = 6,    /* valid request this build does not implement */
+  BBTS_E_FORMAT = 7          /* block format id is not the one the kernel expects */
+};
+
+/* Precision modes of the GEMM / contraction entry points. */
+enum bbts_precision {
+  BBTS_PREC_FP32 = 0,        /* FP32 MFMA, the drop-in */
+  BBTS_PREC_BF16X1 = 1,      /* operands rounded once to BF16, FP32 accumulate */
+  BBTS_PREC_BF16X3 = 2,      /* hi+lo BF16 split, 3 products, FP32 accumulate */
+  BBTS_PREC_BF16X6 = 3,      /* hi+mid+lo BF16 split, 6 products, FP32 accumulate */
+  BBTS_PREC_AUTO = 4,        /* fewest BF16 slices that represent each operand exactly, max 3 */
+  BBTS_PREC_FP8X1 = 5,       /* FP8 e4m3 with a power-of-two scale per block */
+  BBTS_PREC_FP8X3 = 6        /* FP8 e4m3 + one refinement step (3 scaled products) */
+};
+
+/* ---- lifecycle (node start-up: src/server/node.cc:9-54) ------------------ */
+int bbts_hip_init(int device, bbts_hip_ctx **ctx);
+int bbts_hip_shutdown(bbts_hip_ctx *ctx);
+const char *bbts_hip_strerror(int code);
+const char *bbts_hip_last_error(void);
+int bbts_hip_sync(bbts_hip_ctx *ctx, bbts_stream_t stream);
+int bbts_hip_device_info(bbts_hip_ctx *ctx, int *num_cus, int *num_xcds, size_t *hbm_bytes);
+
+/* ---- blocks (storage: src/storage/memory_storage.cc:70-103, 141-157) ----- */
+/* Sizes follow the reference's formats: dense = 260 + 4*rows*cols
+ * (src/tensor/builtin_formats.cc get_size), cutensor = 260 + 4*prod(dims)
+ * (libraries/cutensor/cu.h:84-98). */
+size_t bbts_hip_dense_size(uint32_t rows, uint32_t cols);
+/* Allocate `bytes` (>= 260) in HBM and write the 260 header bytes `meta`. */
+int bbts_hip_block_alloc(bbts_hip_ctx *ctx, const void *meta, size_t bytes, bbts_tensor_t **blk);
+int bbts_hip_block_free(bbts_hip_ctx *ctx, bbts_tensor_t *blk);
+/* Adopt caller-owned device memory [hdr, hdr+bytes) as a block (no copy of the header is made
+ * to the device; `meta` must equal the bytes at hdr).  Free with bbts_hip_block_free (no-op on memory). */
+int bbts_hip_block_register(bbts_hip_ctx *ctx, void *hdr, size_t bytes, const void *meta, bbts_tensor_t **blk);
+int bbts_hip_block_meta(bbts_hip_ctx *ctx, const bbts_tensor_t *blk, void *meta_out, size_t *bytes_out);
+int bbts_hip_block_set_meta(bbts_hip_ctx *ctx, bbts_tensor_t *blk, const void *meta, bbts_stream_t stream);
+void *bbts_hip_block_data(const bbts_tensor_t *blk);
+int bbts_hip_block_upload(bbts_hip_ctx *ctx, bbts_tensor_t *blk, const void *host, size_t bytes, bbts_stream_t stream);
+int bbts_hip_block_download(bbts_hip_ctx *ctx, const bbts_tensor_t *blk, void *host, size_t bytes, bbts_stream_t stream);
+/* Bytes held by the block pool (in use, cached). */
+int bbts_hip_pool_stats(bbts_hip_ctx *ctx, size_t *in_use, size_t *cached);
+
+/* ---- dense ud functions --------------------------------------------------- */
+/* matrix_mult: replaces dense_matrix_mult_t::mult (src/ud_functions/impls/dense_matrix_mult.cc:50-86)
+ * and dense_matrix_gpu_mult_t::mult (dense_matrix_gpu_mult.cu:51-85).  out = op(a)·op(b), row-major,
+ * with the CPU impl's semantics (A·B; params bool 0/1 = transpose a/b, only for square operands);
+ * writes out's meta {a.rows, b.cols}. */
+int bbts_hip_dense_matmul(bbts_hip_ctx *ctx, const bbts_tensor_t *a, const bbts_tensor_t *b, bbts_tensor_t *out,
+                          int trans_a, int trans_b, int precision, bbts_stream_t stream);
+/* matrix_add: replaces dense_matrix_add_t::add (dense_matrix_add.cc:46-77) and
+ * dense_matrix_gpu_add_t::add (dense_matrix_gpu_add.cu:57-85).  out may alias a or b. */
+int bbts_hip_dense_add(bbts_hip_ctx *ctx, const bbts_tensor_t *a, const bbts_tensor_t *b, bbts_tensor_t *out,
+                       bbts_stream_t stream);
+/* Local REDUCE of k blocks with matrix_add: replaces local_reduce_op_t::apply
+ * (src/operations/local_reduce_op.cc:23-91) and reduce_op_t::apply_preagg (reduce_op.cc:196-272).
+ * One pass, the same left-fold order ((in0+in1)+in2)+..., bit-identical to the pairwise chain. */
+int bbts_hip_dense_reduce(bbts_hip_ctx *ctx, const bbts_tensor_t *const *in, int k, bbts_tensor_t *out,
+                          bbts_stream_t stream);
+/* BROADCAST on one GPU: replaces broadcast_op_t::apply (src/operations/broadcast_op.cc:33-120);
+ * each dst receives the whole tensor (header + payload) byte for byte. */
+int bbts_hip_broadcast(bbts_hip_ctx *ctx, const bbts_tensor_t *src, bbts_tensor_t *const *dst, int n,
+                       bbts_stream_t stream);
+/* ReLU: applications/fnn/ffnn_relu.cc:46-71 (out = v > 0 ? v : 0). */
+int bbts_hip_dense_relu(bbts_hip_ctx *ctx, const bbts_tensor_t *a, bbts_tensor_t *out, bbts_stream_t stream);
+/* uniform: replaces dense_uniform_t (src/ud_functions/impls/dense_uniform.cc:27-79) with a
+ * seeded counter-based generator; writes meta {rows, cols} and U[left, right) values. */
+int bbts_hip_dense_uniform(bbts_hip_ctx *ctx, bbts_tensor_t *out, uint32_t rows, uint32_t cols, float left,
+                           float right, uint64_t seed, bbts_stream_t stream);
+
+/* ---- contraction (libraries/cutensor/contraction.h:414-469, 596-624) ------ */
+/* params: the reference's int32 words [rank_l, rank_r, rank_o, modes_l..., modes_r..., modes_o...,
+ * alpha (float bits)]; blocks use the cutensor format {u32 rank; i64 dims[4]} (cu.h:60-63),
+ * column-major payloads.  Writes the out meta (set_out_meta, :58-63). */
+int bbts_hip_contraction(bbts_hip_ctx *ctx, const bbts_tensor_t *lhs, const bbts_tensor_t *rhs, bbts_tensor_t *out,
+                         const int32_t *params, int nparams, int precision, bbts_stream_t stream);
+/* The same on raw column-major device buffers. */
+int bbts_hip_contraction_raw(bbts_hip_ctx *ctx, const int32_t *params, int nparams, int rank_l,
+                             const int64_t *dims_l, const float *lhs, int rank_r, const int64_t *dims_r,
+                             const float *rhs, float *out, int precision, bbts_stream_t stream);
+
+/* ---- raw-pointer kernels (the kernels behind the block entry points) ----- */
+/* C[m*ldc+n] = alpha * sum_k op(A)[m,k] op(B)[k,n]; row-major; ta: A stored K x M (lda >= M);
+ * tb: B stored N x K (ldb >= K).  Batched over `batch` with element strides sa/sb/sc. */
+int bbts_hip_gemm(bbts_hip_ctx *ctx, int M, int N, int K, const float *A, int64_t lda, int ta, const float *B,
+                  int64_t ldb, int tb, float *C, int64_t ldc, float alpha, int precision, bbts_stream_t stream);
+int bbts_hip_gemm_batched(bbts_hip_ctx *ctx, int M, int N, int K, const float *A, int64_t lda, int64_t sa, int ta,
+                          const float *B, int64_t ldb, int64_t sb, int tb, float *C, int64_t ldc, int64_t sc,
+                          int batch, float alpha, int precision, bbts_stream_t stream);
+int bbts_hip_add_f32(bbts_hip_ctx *ctx, const float *a, const float *b, float *out, size_t n, bbts_stream_t stream);
+int bbts_hip_reduce_f32(bbts_hip_ctx *ctx, const float *const *in, int k, float *out, size_t n,
+                        bbts_stream_t stream);
+int bbts_hip_relu_f32(bbts_hip_ctx *ctx, const float *in, float *out, size_t n, bbts_stream_t stream);
+int bbts_hip_copy_n(bbts_hip_ctx *ctx, const void *src, size_t bytes, void *const *dst, int n, bbts_stream_t stream);
+int bbts_hip_gen_uniform(bbts_hip_ctx *ctx, float *out, size_t n, uint64_t seed, uint64_t offset, float left,
+                         float right, bbts_stream_t stream);
+int bbts_hip_gen_normal(bbts_hip_ctx *ctx, float *out, size_t n, uint64_t seed, uint64_t offset, double mean,
+                        double stdev, bbts_stream_t stream);
+/* Converts FP32 to BF16 (round to nearest even). */
+int bbts_hip_f32_to_bf16(bbts_hip_ctx *ctx, const float *in, uint16_t *out, size_t n, bbts_stream_t stream);
+/* FP64 residual check: out2[0] = sum (C - A·B)^2, out2[1] = sum (A·B)^2 over the M x N product,
+ * with A·B formed in FP64 on the GPU (row-major operands; a_bf16/b_bf16 select BF16 operand storage).
+ * out2 is a device pointer to 2 doubles, overwritten. */
+int bbts_hip_residual_f64(bbts_hip_ctx *ctx, int M, int N, int K, const void *A, int64_t lda, int a_bf16,
+                          const void *B, int64_t ldb, int b_bf16, const float *C, int64_t ldc, double *out2,
+                          bbts_stream_t stream);
+
+/* ---- batching: grouped persistent GEMM launches over a block-op stream --- */
+typedef struct {
+  int32_t M, N, K;
+  int32_t dtype;           /* 0 = FP32 operands, 1 = BF16 operands (uint16 storage) */
+  const void *A;           /* row-major M x K, leading dimension lda */
+  const void *B;           /* row-major K x N, leading dimension ldb */
+  float *C;                /* row-major M x N FP32, leading dimension ldc */
+  int64_t lda, ldb, ldc;
+} bbts_gemm_desc_t;
+
+enum bbts_schedule_policy {
+  BBTS_SCHED_FIFO = 0,     /* submission order, one default tile */
+  BBTS_SCHED_HEURISTIC = 1,/* size heuristic for tiles, largest ops first */
+  BBTS_SCHED_TUNED = 2,    /* tile per op from the autotuner's table, largest-first by measured time */
+  BBTS_SCHED_LEARNED = 3,  /* tile per op and launch order from the cost model's predictions */
+  BBTS_SCHED_FORCED_CFG0 = 100 /* 100 + c: every op on tile config c, submission order */
+};
+
+int bbts_hip_batch_create(bbts_hip_ctx *ctx, bbts_batch **batch);
+int bbts_hip_batch_destroy(bbts_batch *batch);
+int bbts_hip_batch_clear(bbts_batch *batch);
+int bbts_hip_batch_add_gemm(bbts_batch *batch, const bbts_gemm_desc_t *desc, int precision);
+/* Plan tiles and tile order; cm may be NULL unless policy is BBTS_SCHED_LEARNED. */
+int bbts_hip_batch_plan(bbts_batch *batch, int policy, bbts_costmodel *cm);
+/* Launch the planned stream (one grouped persistent launch per tile shape). */
+int bbts_hip_batch_launch(bbts_batch *batch, bbts_stream_t stream);
+/* Capture the planned launches into a hipGraph once; replay it. */
+int bbts_hip_batch_capture(bbts_batch *batch, bbts_stream_t stream);
+int bbts_hip_batch_replay(bbts_batch *batch, bbts_stream_t stream);
+/* Number of tile configurations, the config chosen for op i, and per-config shape. */
+int bbts_hip_num_tile_configs(void);
+int bbts_hip_tile_config_shape(int cfg, int *bm, int *bn, int *bk);
+int bbts_hip_batch_op_config(bbts_batch *batch, int op, int *cfg);
+/* Planned launches: one per (tile config, operand variant); group g's config, work items,
+ * algorithmic FP32-equivalent flops (2*M*N*K summed over its ops), and a launch of that group alone. */
+int bbts_hip_batch_num_groups(bbts_batch *batch);
+int bbts_hip_batch_group_info(bbts_batch *batch, int g, int *cfg, int *nitems, int *grid, double *flops);
+int bbts_hip_batch_launch_group(bbts_batch *batch, int g, bbts_stream_t stream);
+/* Plain per-op launch of one GEMM descriptor with a given tile config (used by the autotuner). */
+int bbts_hip_gemm_desc_launch(bbts_hip_ctx *ctx, const bbts_gemm_desc_t *desc, int precision, int cfg,
+                              bbts_stream_t stream);
+
+/* ---- autotuner: exhaustive timing of every tile config per GEMM shape ---- */
+/* Times each config for each distinct (M,N,K,dtype,precision) in the batch with HIP events
+ * (reps timed launches after one warm-up); fills the tuning table of the context and
+ * writes per-(op,config) microseconds into times_us[nops * num_tile_configs] (may be NULL). */
+int bbts_hip_tune_batch(bbts_batch *batch, int reps, float *times_us);
+/* Tuning-table lookup (cfg = -1 if not tuned). */
+int bbts_hip_tune_lookup(bbts_hip_ctx *ctx, int M, int N, int K, int dtype, int precision, int *cfg, float *us);
+
+/* ---- learned cost model: MLP trained and evaluated by this library's kernels */
+/* Features per (GEMM, tile config): computed by bbts_hip_costmodel_features. */
+int bbts_hip_costmodel_create(bbts_hip_ctx *ctx, int hidden, uint64_t seed, bbts_costmodel **cm);
+int bbts_hip_costmodel_destroy(bbts_costmodel *cm);
+int bbts_hip_costmodel_num_features(void);
+/* Host-side feature vector of one (M, N, K, dtype, precision, cfg) sample. */
+int bbts_hip_costmodel_features(int M, int N, int K, int dtype, int precision, int cfg, float *feat);
+/* One epoch of minibatch SGD (Adam) on device samples X[n, nf], y[n] (log-microseconds);
+ * loss_out (host) receives the mean squared error before the update. */
+int bbts_hip_costmodel_train(bbts_costmodel *cm, const float *X, const float *y, int n, int batch, float lr,
+                             int epochs, float *loss_out, bbts_stream_t stream);
+int bbts_hip_costmodel_predict(bbts_costmodel *cm, const float *X, int n, float *y, bbts_stream_t stream);
+/* Forward + backward of the mean squared loss on one batch without updating, for gradient checks:
+ * grads (device) receives the flattened parameter gradient, loss (device) the loss. */
+int bbts_hip_costmodel_grad(bbts_costmodel *cm, const float *X, const float *y, int n, float *grads, float *loss,
+                            bbts_stream_t stream);
+int bbts_hip_costmodel_num_params(bbts_costmodel *cm);
+int bbts_hip_costmodel_get_params(bbts_costmodel *cm, float *host, int n);
+int bbts_hip_costmodel_set_params(bbts_costmodel *cm, const float *host, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BBTS_HIP_H */

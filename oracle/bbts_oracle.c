@@ -1,0 +1,566 @@
+/*
+ * bbts_oracle.c -- CPU restatement of the reference's per-node block kernels.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: tests/, the smoke()
+ * entry and bench.py's cpu_baseline leg may load it; the product library
+ * (bbts_amd/, libbbts_hip.so) never links, loads or calls it.
+ *
+ * Every function states the reference file:line whose semantics it restates
+ * (paths relative to the dcbdan/bbts source tree).  Nothing here is copied:
+ * loops are written from the semantics, in plain C, compiled with
+ * -ffp-contract=off so that FP32 sums are rounded exactly as written.
+ *
+ * How each function is pinned (beyond restating the reference):
+ *   orc_dense_size ........ size law of tests/test_builtin_dense.cc:27-38
+ *   orc_dense_add ......... closed form of tests/test_builtin_ud.cc:46-73
+ *   orc_local_reduce ...... integer-valued sums of integration_tests/test_reduce.cc:101-111,
+ *                           and k=2 equals orc_dense_add bit for bit
+ *   orc_broadcast ......... byte-exact memcmp (integration_tests/test_broadcast.cc:91-93)
+ *   orc_gemm_f64 .......... brute force on tiny shapes, identity / permutation /
+ *                           rank-1 / integer-valued closed forms, numpy matmul
+ *   orc_contraction_* ..... brute force einsum restated from reference()
+ *                           (libraries/cutensor/contraction.h:131-202), numpy.einsum
+ *                           in FP64, GEMM special cases
+ *   orc_rng_* ............. distribution moments; exact closed form of the counter mix
+ * The "parity unpinned" items are listed in the Python wrapper's header.
+ */
+#include <dlfcn.h>
+#include <math.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_META_BYTES 260          /* sizeof(tensor_meta_t): src/tensor/tensor.h:20-34 */
+#define ORC_BLOB_OFF 4              /* offsetof(tensor_meta_t, _blob) */
+#define ORC_MAXRANK 4               /* libraries/cutensor/cu.h:16 */
+
+/* ------------------------------------------------------------------------- */
+/* Block header layout (src/tensor/tensor.h:20-52, builtin_formats.h:7-45,   */
+/* libraries/cutensor/cu.h:60-99).  Written byte by byte, no structs.        */
+/* ------------------------------------------------------------------------- */
+
+/* size of a dense block: builtin_formats.cc get_size lambda
+ * (sizeof(tensor_meta_t) + rows*cols*sizeof(float)). */
+size_t orc_dense_size(uint32_t rows, uint32_t cols) {
+  return (size_t)ORC_META_BYTES + (size_t)rows * (size_t)cols * 4u;
+}
+
+/* dense header: int32 fmt_id at +0, {u32 rows, u32 cols} at +4 (blob start),
+ * the other 248 blob bytes are left as given (this build zeroes them). */
+void orc_dense_header(uint8_t *hdr, int32_t fmt_id, uint32_t rows, uint32_t cols) {
+  memset(hdr, 0, ORC_META_BYTES);
+  memcpy(hdr + 0, &fmt_id, 4);
+  memcpy(hdr + ORC_BLOB_OFF + 0, &rows, 4);
+  memcpy(hdr + ORC_BLOB_OFF + 4, &cols, 4);
+}
+
+/* cutensor header (cu.h:60-63): struct cu_shape_t { uint32_t rank; int64_t dims[4]; }
+ * placed at the blob start; natural alignment puts dims at blob+8, i.e. header+12. */
+void orc_cu_header(uint8_t *hdr, int32_t fmt_id, uint32_t rank, const int64_t *dims) {
+  memset(hdr, 0, ORC_META_BYTES);
+  memcpy(hdr + 0, &fmt_id, 4);
+  memcpy(hdr + ORC_BLOB_OFF, &rank, 4);
+  for (uint32_t r = 0; r < rank && r < ORC_MAXRANK; ++r)
+    memcpy(hdr + ORC_BLOB_OFF + 8 + 8 * r, &dims[r], 8);
+}
+
+/* cu_t size (cu.h:84-98, 123-127). */
+size_t orc_cu_size(uint32_t rank, const int64_t *dims) {
+  size_t n = 1;
+  for (uint32_t r = 0; r < rank; ++r) n *= (size_t)dims[r];
+  return (size_t)ORC_META_BYTES + 4u * n;
+}
+
+/* Complexity hints, restated with the reference's uint32 arithmetic:
+ * dense_matrix_mult.cc:26-33 returns m_a.rows * m_a.cols * m_b.cols computed
+ * in uint32_t (wraps for 2048^3); dense_matrix_add.cc:110-116 returns rows*cols. */
+uint64_t orc_mult_hint(uint32_t a_rows, uint32_t a_cols, uint32_t b_cols) {
+  uint32_t h = a_rows * a_cols * b_cols; /* wraps mod 2^32 like the reference */
+  return (uint64_t)h;
+}
+uint64_t orc_add_hint(uint32_t rows, uint32_t cols) { return (uint64_t)(uint32_t)(rows * cols); }
+
+/* ------------------------------------------------------------------------- */
+/* Seeded counter-based generator.  The reference seeds MKL VSL with time()   */
+/* (src/ud_functions/impls/dense_uniform.cc:50-79), which is not reproducible; */
+/* this build replaces it by splitmix64 (S. Vigna, public domain, published   */
+/* 2015) evaluated at counter i, shared bit for bit with the HIP generator.   */
+/* ------------------------------------------------------------------------- */
+static inline uint64_t orc_splitmix(uint64_t seed, uint64_t i) {
+  uint64_t z = seed + (i + 1u) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+uint64_t orc_rng_u64(uint64_t seed, uint64_t i) { return orc_splitmix(seed, i); }
+
+/* uniform [left, right): u = top 24 bits * 2^-24, value = left + (right-left)*u
+ * in float with two roundings (dense_uniform.cc params {rows, cols, left, right}). */
+void orc_rng_uniform(float *out, size_t n, uint64_t seed, uint64_t offset, float left, float right) {
+  const float w = right - left;
+  for (size_t i = 0; i < n; ++i) {
+    uint64_t x = orc_splitmix(seed, offset + i);
+    float u = (float)(x >> 40) * (1.0f / 16777216.0f);
+    float p = w * u;
+    out[i] = left + p;
+  }
+}
+
+/* N(mean, std^2) by Box-Muller on the counter pair (2i, 2i+1), in double,
+ * rounded once to float. */
+void orc_rng_normal(float *out, size_t n, uint64_t seed, uint64_t offset, double mean, double stdev) {
+  for (size_t i = 0; i < n; ++i) {
+    uint64_t j = offset + i;
+    uint64_t x1 = orc_splitmix(seed, 2u * j);
+    uint64_t x2 = orc_splitmix(seed, 2u * j + 1u);
+    double u1 = ((double)(x1 >> 11) + 1.0) * (1.0 / 9007199254740992.0); /* (0,1] */
+    double u2 = (double)(x2 >> 11) * (1.0 / 9007199254740992.0);         /* [0,1) */
+    double r = sqrt(-2.0 * log(u1));
+    double z = r * cos(6.283185307179586476925286766559 * u2);
+    out[i] = (float)(mean + stdev * z);
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Dense add (src/ud_functions/impls/dense_matrix_add.cc:132-163 in the ud;  */
+/* the loop is :154-159): out[r*cols+c] = a[...] + b[...] in FP32.            */
+/* ------------------------------------------------------------------------- */
+void orc_dense_add(const float *a, const float *b, float *out, uint32_t rows, uint32_t cols) {
+  for (uint32_t r = 0; r < rows; ++r)
+    for (uint32_t c = 0; c < cols; ++c) {
+      size_t i = (size_t)r * cols + c;
+      out[i] = a[i] + b[i];
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Local reduce (src/operations/local_reduce_op.cc:23-91) and the local       */
+/* pre-aggregation of a distributed reduce (src/operations/reduce_op.cc:196-272): */
+/* lhs = in[0]; for idx = 1..k-1: lhs = add(lhs, in[idx]) -- an ordered left   */
+/* fold ((x0+x1)+x2)+... with one FP32 rounding per add.  k == 1 renames the   */
+/* input (no arithmetic), which here is a copy.                                */
+/* ------------------------------------------------------------------------- */
+void orc_local_reduce(const float *const *ins, int k, float *out, size_t n) {
+  if (k <= 0) return;
+  if (k == 1) { memcpy(out, ins[0], n * sizeof(float)); return; }
+  for (size_t i = 0; i < n; ++i) {
+    float acc = ins[0][i] + ins[1][i];
+    for (int j = 2; j < k; ++j) acc = acc + ins[j][i];
+    out[i] = acc;
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Broadcast (src/operations/broadcast_op.cc:33-120): every destination holds */
+/* the whole tensor (header + payload, num_bytes = get_tensor_size) byte for  */
+/* byte.  On one node the tree collapses to n copies of the source bytes.     */
+/* ------------------------------------------------------------------------- */
+void orc_broadcast(const uint8_t *src, size_t nbytes, uint8_t *const *dsts, int n) {
+  for (int d = 0; d < n; ++d) memcpy(dsts[d], src, nbytes);
+}
+
+/* ReLU (applications/fnn/ffnn_relu.cc:46-71; src/ud_functions/impls/dense_matrix_relu.cc:46-71):
+ * out = v > 0 ? v : 0. */
+void orc_relu(const float *in, float *out, size_t n) {
+  for (size_t i = 0; i < n; ++i) out[i] = in[i] > 0.0f ? in[i] : 0.0f;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Dense matmul (src/ud_functions/impls/dense_matrix_mult.cc:50-86):          */
+/*   cblas_sgemm(RowMajor, ta, tb, I, J, K, 1, A, lda=K, B, ldb=J, 0, C, ldc=J) */
+/* with I = a.rows, J = b.cols, K = a.cols, ta/tb = params bool 0/1 (:78-79).  */
+/* Element access with those leading dimensions:                               */
+/*   op(A)[i][k] = ta ? A[k*K + i] : A[i*K + k]                                 */
+/*   op(B)[k][j] = tb ? B[j*J + k] : B[k*J + j]                                 */
+/* (only in bounds for square operands when transposed, as in the reference). */
+/* FP64 accumulation: this is the exact-arithmetic oracle of that product.     */
+/* ------------------------------------------------------------------------- */
+void orc_gemm_f64(const float *A, const float *B, double *C, uint32_t I, uint32_t J, uint32_t K,
+                  int ta, int tb) {
+  for (uint32_t i = 0; i < I; ++i) {
+    double *crow = C + (size_t)i * J;
+    for (uint32_t j = 0; j < J; ++j) crow[j] = 0.0;
+    for (uint32_t k = 0; k < K; ++k) {
+      double a = ta ? (double)A[(size_t)k * K + i] : (double)A[(size_t)i * K + k];
+      if (!tb) {
+        const float *brow = B + (size_t)k * J;
+        for (uint32_t j = 0; j < J; ++j) crow[j] += a * (double)brow[j];
+      } else {
+        for (uint32_t j = 0; j < J; ++j) crow[j] += a * (double)B[(size_t)j * J + k];
+      }
+    }
+  }
+}
+
+/* FP64 GEMM on double inputs, row-major, C[M,N] = sum_k A[m*lda+k] * B[k*ldb+n].
+ * Used to check split-precision products exactly. */
+void orc_dgemm_rm(const double *A, const double *B, double *C, uint32_t M, uint32_t N, uint32_t K,
+                  uint32_t lda, uint32_t ldb, uint32_t ldc) {
+  for (uint32_t m = 0; m < M; ++m) {
+    double *crow = C + (size_t)m * ldc;
+    for (uint32_t n = 0; n < N; ++n) crow[n] = 0.0;
+    for (uint32_t k = 0; k < K; ++k) {
+      double a = A[(size_t)m * lda + k];
+      const double *brow = B + (size_t)k * ldb;
+      for (uint32_t n = 0; n < N; ++n) crow[n] += a * brow[n];
+    }
+  }
+}
+
+/* ---- FP32 sgemm provider for the CPU baseline ------------------------------
+ * The reference links MKL and calls cblas_sgemm (dense_matrix_mult.cc:82).
+ * Resolve the same routine at run time through the standard CBLAS prototype;
+ * without it, fall back to a blocked loop. */
+typedef void (*orc_cblas_sgemm_t)(int, int, int, int, int, int, float, const float *, int,
+                                  const float *, int, float, float *, int);
+static orc_cblas_sgemm_t orc_sgemm_fn = 0;
+static int orc_sgemm_resolved = 0;
+static char orc_sgemm_name[256] = "loops";
+
+static void orc_resolve_sgemm(void) {
+  if (orc_sgemm_resolved) return;
+  orc_sgemm_resolved = 1;
+  const char *cands[] = {"libmkl_rt.so", "libmkl_rt.so.2", "libmkl_rt.so.1",
+                         "/opt/conda/lib/libmkl_rt.so", "libopenblas.so.0", "libopenblas.so",
+                         "libcblas.so.3", "libblas.so.3", 0};
+  for (int i = 0; cands[i]; ++i) {
+    void *h = dlopen(cands[i], RTLD_NOW | RTLD_LOCAL);
+    if (!h) continue;
+    void *f = dlsym(h, "cblas_sgemm");
+    if (f) {
+      orc_sgemm_fn = (orc_cblas_sgemm_t)f;
+      strncpy(orc_sgemm_name, cands[i], sizeof(orc_sgemm_name) - 1);
+      return;
+    }
+    dlclose(h);
+  }
+}
+
+const char *orc_sgemm_provider(void) {
+  orc_resolve_sgemm();
+  return orc_sgemm_name;
+}
+
+/* The reference's FP32 call, verbatim in meaning (row-major, lda=K, ldb=J, ldc=J). */
+void orc_dense_mult_f32(const float *A, const float *B, float *C, uint32_t I, uint32_t J, uint32_t K,
+                        int ta, int tb) {
+  orc_resolve_sgemm();
+  if (orc_sgemm_fn) {
+    /* CBLAS enums: RowMajor=101, NoTrans=111, Trans=112 */
+    orc_sgemm_fn(101, ta ? 112 : 111, tb ? 112 : 111, (int)I, (int)J, (int)K, 1.0f, A, (int)K, B,
+                 (int)J, 0.0f, C, (int)J);
+    return;
+  }
+  for (uint32_t i = 0; i < I; ++i) {
+    float *crow = C + (size_t)i * J;
+    for (uint32_t j = 0; j < J; ++j) crow[j] = 0.0f;
+    for (uint32_t k = 0; k < K; ++k) {
+      float a = ta ? A[(size_t)k * K + i] : A[(size_t)i * K + k];
+      for (uint32_t j = 0; j < J; ++j) crow[j] += a * (tb ? B[(size_t)j * J + k] : B[(size_t)k * J + j]);
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Contraction (libraries/cutensor/contraction.h).                            */
+/* Params layout (parse, :25-56):                                             */
+/*   [rank_l, rank_r, rank_o, modes_l..., modes_r..., modes_o..., alpha(f32)]  */
+/* encoded as int32 words; alpha's word holds the float's bits.                */
+/* Tensors are column-major (first mode fastest, cu.h:155-163).                */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  int nmodes;               /* number of incident modes (max mode id + 1) */
+  int64_t dims[16];         /* dims per incident mode */
+  int rl, rr, ro;
+  int ml[ORC_MAXRANK], mr[ORC_MAXRANK], mo[ORC_MAXRANK];
+  float alpha;
+} orc_cinfo_t;
+
+/* parse (:25-56): incident dims are the max mode id + 1 over lhs and rhs;
+ * dims come from the lhs then the rhs shapes (rhs overwrites). Returns 0 on success. */
+int orc_contraction_parse(const int32_t *params, int nparams, uint32_t rank_l, const int64_t *dims_l,
+                          uint32_t rank_r, const int64_t *dims_r, orc_cinfo_t *info) {
+  if (nparams < 3) return -1;
+  int rl = params[0], rr = params[1], ro = params[2];
+  if (rl < 0 || rr < 0 || ro < 0 || rl > ORC_MAXRANK || rr > ORC_MAXRANK || ro > ORC_MAXRANK) return -1;
+  if (nparams < 3 + rl + rr + ro + 1) return -1;
+  info->rl = rl; info->rr = rr; info->ro = ro;
+  int m = 0, i = 3;
+  for (int x = 0; x < rl; ++x, ++i) { info->ml[x] = params[i]; if (1 + params[i] > m) m = 1 + params[i]; }
+  for (int x = 0; x < rr; ++x, ++i) { info->mr[x] = params[i]; if (1 + params[i] > m) m = 1 + params[i]; }
+  for (int x = 0; x < ro; ++x, ++i) info->mo[x] = params[i];
+  memcpy(&info->alpha, &params[i], 4);
+  if (m > 16) return -1;
+  info->nmodes = m;
+  for (int x = 0; x < m; ++x) info->dims[x] = 0;
+  if ((int)rank_l != rl || (int)rank_r != rr) return -1;
+  for (int x = 0; x < rl; ++x) info->dims[info->ml[x]] = dims_l[x];
+  for (int x = 0; x < rr; ++x) info->dims[info->mr[x]] = dims_r[x];
+  for (int x = 0; x < ro; ++x) if (info->mo[x] < 0 || info->mo[x] >= m) return -1;
+  return 0;
+}
+
+/* set_out_meta (:58-63): out dims in m_out order. */
+void orc_contraction_out_dims(const orc_cinfo_t *info, int64_t *out_dims) {
+  for (int r = 0; r < info->ro; ++r) out_dims[r] = info->dims[info->mo[r]];
+}
+
+/* complexity hint (:644-653): product of all incident dims. */
+int64_t orc_contraction_hint(const orc_cinfo_t *info) {
+  int64_t p = 1;
+  for (int x = 0; x < info->nmodes; ++x) p *= info->dims[x];
+  return p;
+}
+
+/* offset of an incident index tuple in a column-major tensor whose modes are
+ * `ordering` (utils.h:94-130 get_offset / get_offset_wrt_ordering). */
+static int64_t orc_offset(const int64_t *dims, const int64_t *idx, const int *ordering, int rank) {
+  int64_t s = 1, off = 0;
+  for (int r = 0; r < rank; ++r) { off += s * idx[ordering[r]]; s *= dims[ordering[r]]; }
+  return off;
+}
+
+/* Brute force (reference(), :131-202): for every incident index tuple,
+ * out[o] += lhs[l] * rhs[r]; then out *= alpha.  Accumulated in FP64 here. */
+int orc_contraction_brute(const orc_cinfo_t *info, const float *lhs, const float *rhs, double *out) {
+  int64_t nout = 1;
+  for (int r = 0; r < info->ro; ++r) nout *= info->dims[info->mo[r]];
+  for (int64_t i = 0; i < nout; ++i) out[i] = 0.0;
+  int n = info->nmodes;
+  int64_t idx[16] = {0};
+  int64_t total = 1;
+  for (int x = 0; x < n; ++x) total *= info->dims[x];
+  if (total == 0) return 0;
+  for (int64_t t = 0; t < total; ++t) {
+    int64_t rem = t;
+    for (int x = 0; x < n; ++x) { idx[x] = rem % info->dims[x]; rem /= info->dims[x]; }
+    int64_t o = orc_offset(info->dims, idx, info->mo, info->ro);
+    int64_t l = orc_offset(info->dims, idx, info->ml, info->rl);
+    int64_t r = orc_offset(info->dims, idx, info->mr, info->rr);
+    out[o] += (double)lhs[l] * (double)rhs[r];
+  }
+  for (int64_t i = 0; i < nout; ++i) out[i] *= (double)info->alpha;
+  return 0;
+}
+
+/* mode classes (cpu_op::which_t, :229-253) */
+enum { ORC_I = 0, ORC_J, ORC_K, ORC_L, ORC_R, ORC_B, ORC_BAD };
+static int orc_mode_class(const orc_cinfo_t *info, int m) {
+  int left = 0, right = 0, out = 0;
+  for (int x = 0; x < info->rl; ++x) left |= info->ml[x] == m;
+  for (int x = 0; x < info->rr; ++x) right |= info->mr[x] == m;
+  for (int x = 0; x < info->ro; ++x) out |= info->mo[x] == m;
+  if (left && !right && out) return ORC_I;
+  if (left && right && !out) return ORC_J;
+  if (!left && right && out) return ORC_K;
+  if (left && !right && !out) return ORC_L;
+  if (!left && right && !out) return ORC_R;
+  if (left && right && out) return ORC_B;
+  return ORC_BAD;
+}
+
+typedef struct {
+  int permute_lhs[ORC_MAXRANK], permute_rhs[ORC_MAXRANK], permute_out[ORC_MAXRANK];
+  int64_t nb, ni, nj, nk, nl, nr;
+  int t_lhs, t_rhs, t_out;
+} orc_cplan_t;
+
+/* cpu_op::get_plan (:267-377). */
+int orc_contraction_plan(const orc_cinfo_t *info, orc_cplan_t *plan) {
+  int n = info->nmodes;
+  int cls[16];
+  plan->nb = plan->ni = plan->nj = plan->nk = plan->nl = plan->nr = 1;
+  for (int x = 0; x < n; ++x) {
+    cls[x] = orc_mode_class(info, x);
+    switch (cls[x]) {
+      case ORC_I: plan->ni *= info->dims[x]; break;
+      case ORC_J: plan->nj *= info->dims[x]; break;
+      case ORC_K: plan->nk *= info->dims[x]; break;
+      case ORC_L: plan->nl *= info->dims[x]; break;
+      case ORC_R: plan->nr *= info->dims[x]; break;
+      case ORC_B: plan->nb *= info->dims[x]; break;
+      default: return -1; /* "should not happen" */
+    }
+  }
+  int ig[16], jg[16], kg[16], bg[16], lg[16], rg[16];
+  int ni = 0, nj = 0, nk = 0, nb = 0, nl = 0, nr = 0;
+  for (int x = 0; x < info->ro; ++x) {
+    int m = info->mo[x];
+    if (cls[m] == ORC_I) ig[ni++] = m; else if (cls[m] == ORC_K) kg[nk++] = m; else if (cls[m] == ORC_B) bg[nb++] = m;
+  }
+  for (int x = 0; x < info->rl; ++x) {
+    int m = info->ml[x];
+    if (cls[m] == ORC_J) jg[nj++] = m; else if (cls[m] == ORC_L) lg[nl++] = m;
+  }
+  for (int x = 0; x < info->rr; ++x) { int m = info->mr[x]; if (cls[m] == ORC_R) rg[nr++] = m; }
+
+  /* lhs */
+  {
+    int inv[16];
+    for (int x = 0; x < 16; ++x) inv[x] = -1;
+    for (int x = 0; x < info->rl; ++x) inv[info->ml[x]] = x;
+    plan->t_lhs = 0;
+    for (int x = 0; x < info->rl; ++x) {
+      int m = info->ml[x];
+      if (cls[m] == ORC_I) break;
+      if (cls[m] == ORC_J) { plan->t_lhs = 1; break; }
+    }
+    int c = 0;
+    if (plan->t_lhs) { for (int y = 0; y < nj; ++y) plan->permute_lhs[c++] = inv[jg[y]]; for (int y = 0; y < ni; ++y) plan->permute_lhs[c++] = inv[ig[y]]; }
+    else             { for (int y = 0; y < ni; ++y) plan->permute_lhs[c++] = inv[ig[y]]; for (int y = 0; y < nj; ++y) plan->permute_lhs[c++] = inv[jg[y]]; }
+    for (int y = 0; y < nb; ++y) plan->permute_lhs[c++] = inv[bg[y]];
+    for (int y = 0; y < nl; ++y) plan->permute_lhs[c++] = inv[lg[y]];
+  }
+  /* rhs */
+  {
+    int inv[16];
+    for (int x = 0; x < 16; ++x) inv[x] = -1;
+    for (int x = 0; x < info->rr; ++x) inv[info->mr[x]] = x;
+    plan->t_rhs = 0;
+    for (int x = 0; x < info->rr; ++x) {
+      int m = info->mr[x];
+      if (cls[m] == ORC_J) break;
+      if (cls[m] == ORC_K) { plan->t_rhs = 1; break; }
+    }
+    int c = 0;
+    if (plan->t_rhs) { for (int y = 0; y < nk; ++y) plan->permute_rhs[c++] = inv[kg[y]]; for (int y = 0; y < nj; ++y) plan->permute_rhs[c++] = inv[jg[y]]; }
+    else             { for (int y = 0; y < nj; ++y) plan->permute_rhs[c++] = inv[jg[y]]; for (int y = 0; y < nk; ++y) plan->permute_rhs[c++] = inv[kg[y]]; }
+    for (int y = 0; y < nb; ++y) plan->permute_rhs[c++] = inv[bg[y]];
+    for (int y = 0; y < nr; ++y) plan->permute_rhs[c++] = inv[rg[y]];
+  }
+  /* out */
+  {
+    int inv[16];
+    for (int x = 0; x < 16; ++x) inv[x] = -1;
+    for (int x = 0; x < info->ro; ++x) inv[info->mo[x]] = x;
+    plan->t_out = 0;
+    for (int x = 0; x < info->ro; ++x) {
+      int m = info->mo[x];
+      if (cls[m] == ORC_I) break;
+      if (cls[m] == ORC_K) { plan->t_out = 1; break; }
+    }
+    for (int x = 0; x < info->ro; ++x) plan->permute_out[x] = inv[info->mo[x]];
+  }
+  return 0;
+}
+
+/* One column-major batched transpose: the data is viewed as [nrow, ncol, batch]
+ * with modes [0..x] forming the rows, (x..y] the columns and (y..] the batch,
+ * and becomes [ncol, nrow, batch] (mkl_?omatcopy_batch_strided('c','t'),
+ * utils.h:206-252). */
+static void orc_swap_modes(const int64_t *dims, int rank, int x, int y, float *buf, float *tmp) {
+  if (x > y) { int t = x; x = y; y = t; }
+  int64_t batch = 1, ncol = 1, nrow = 1;
+  int r = rank - 1;
+  for (; r != y; --r) batch *= dims[r];
+  for (; r != x; --r) ncol *= dims[r];
+  for (; r >= 0; --r) nrow *= dims[r];
+  int64_t ms = nrow * ncol;
+  for (int64_t b = 0; b < batch; ++b) {
+    const float *s = buf + b * ms;
+    float *d = tmp + b * ms;
+    for (int64_t c = 0; c < ncol; ++c)
+      for (int64_t rr = 0; rr < nrow; ++rr) d[c + rr * ncol] = s[rr + c * nrow];
+  }
+  memcpy(buf, tmp, (size_t)(ms * batch) * sizeof(float));
+}
+
+/* permute_t (utils.h:168-204): repeatedly bring the largest remaining label of
+ * `ordering` to the last open slot with one batched transpose, rotating dims
+ * and labels the same way, until the labels are sorted.  Operates in place on
+ * buf (n elements) using tmp as scratch. */
+void orc_permute(const int64_t *dims_in, const int *ordering_in, int rank, float *buf, float *tmp) {
+  int64_t dims[ORC_MAXRANK];
+  int ord[ORC_MAXRANK];
+  for (int r = 0; r < rank; ++r) { dims[r] = dims_in[r]; ord[r] = ordering_in[r]; }
+  for (int r = rank - 1; r > 0; --r) {
+    int which = r, score = ord[r];
+    for (int j = 0; j != r; ++j) if (ord[j] > score) { which = j; score = ord[j]; }
+    if (r == which) continue;
+    orc_swap_modes(dims, rank, r, which, buf, tmp);
+    /* positions [0..which] move after (which..r]: forward rotation by which+1 */
+    int64_t d2[ORC_MAXRANK]; int o2[ORC_MAXRANK];
+    int c = 0;
+    for (int q = which + 1; q <= r; ++q) { d2[c] = dims[q]; o2[c] = ord[q]; ++c; }
+    for (int q = 0; q <= which; ++q) { d2[c] = dims[q]; o2[c] = ord[q]; ++c; }
+    for (int q = 0; q <= r; ++q) { dims[q] = d2[q]; ord[q] = o2[q]; }
+  }
+}
+
+/* cpu_op::operator() (:414-469) restated: permute lhs and rhs by the plan,
+ * sum out l / r modes (reduce_t, :379-412, left fold over the slowest
+ * group), then the column-major batched product (here in FP64 with alpha),
+ * then the in-place output permute (always the identity ordering, because
+ * permute_out[x] = inverse[m_out[x]] = x).  `out` receives FP64 values in the
+ * layout the reference's batched GEMM writes. */
+int orc_contraction_cpu_op(const orc_cinfo_t *info, const float *lhs, const float *rhs, double *out) {
+  orc_cplan_t p;
+  if (orc_contraction_plan(info, &p) != 0) return -1;
+  int64_t dl[ORC_MAXRANK], dr[ORC_MAXRANK];
+  int64_t nlhs = 1, nrhs = 1;
+  for (int x = 0; x < info->rl; ++x) { dl[x] = info->dims[info->ml[x]]; nlhs *= dl[x]; }
+  for (int x = 0; x < info->rr; ++x) { dr[x] = info->dims[info->mr[x]]; nrhs *= dr[x]; }
+  float *L = (float *)malloc(sizeof(float) * (size_t)(nlhs ? nlhs : 1));
+  float *R = (float *)malloc(sizeof(float) * (size_t)(nrhs ? nrhs : 1));
+  float *T = (float *)malloc(sizeof(float) * (size_t)((nlhs > nrhs ? nlhs : nrhs) ? (nlhs > nrhs ? nlhs : nrhs) : 1));
+  memcpy(L, lhs, sizeof(float) * (size_t)nlhs);
+  memcpy(R, rhs, sizeof(float) * (size_t)nrhs);
+  orc_permute(dl, p.permute_lhs, info->rl, L, T);
+  orc_permute(dr, p.permute_rhs, info->rr, R, T);
+  /* reduce_t: sum the nl (nr) slices of size ni*nj*nb (nj*nk*nb), left fold in FP32 */
+  int64_t sl = p.ni * p.nj * p.nb, sr = p.nj * p.nk * p.nb;
+  for (int64_t a = 1; a < p.nl; ++a) for (int64_t e = 0; e < sl; ++e) L[e] = L[e] + L[a * sl + e];
+  for (int64_t a = 1; a < p.nr; ++a) for (int64_t e = 0; e < sr; ++e) R[e] = R[e] + R[a * sr + e];
+  double alpha = (double)info->alpha;
+  for (int64_t b = 0; b < p.nb; ++b) {
+    const float *Lb = L + b * p.ni * p.nj;
+    const float *Rb = R + b * p.nj * p.nk;
+    double *Ob = out + b * p.ni * p.nk;
+    for (int64_t i = 0; i < p.ni; ++i)
+      for (int64_t k = 0; k < p.nk; ++k) {
+        double s = 0.0;
+        for (int64_t j = 0; j < p.nj; ++j) {
+          /* lhs is [i,j] (t_lhs: [j,i]) col-major; rhs is [j,k] (t_rhs: [k,j]) */
+          double a = p.t_lhs ? Lb[j + i * p.nj] : Lb[i + j * p.ni];
+          double c = p.t_rhs ? Rb[k + j * p.nk] : Rb[j + k * p.nj];
+          s += a * c;
+        }
+        /* t_out writes [k,i] col-major (the swapped call, :449-461), else [i,k] */
+        if (p.t_out) Ob[k + i * p.nk] = alpha * s; else Ob[i + k * p.ni] = alpha * s;
+      }
+  }
+  free(L); free(R); free(T);
+  return 0;
+}
+
+/* Helpers for the Python wrapper. */
+int orc_contraction_info(const int32_t *params, int nparams, uint32_t rank_l, const int64_t *dims_l,
+                         uint32_t rank_r, const int64_t *dims_r, int64_t *out_dims, int32_t *out_rank,
+                         int64_t *plan6, int32_t *flags3) {
+  orc_cinfo_t info;
+  if (orc_contraction_parse(params, nparams, rank_l, dims_l, rank_r, dims_r, &info)) return -1;
+  orc_contraction_out_dims(&info, out_dims);
+  *out_rank = info.ro;
+  orc_cplan_t p;
+  if (orc_contraction_plan(&info, &p)) return -2;
+  plan6[0] = p.ni; plan6[1] = p.nj; plan6[2] = p.nk; plan6[3] = p.nb; plan6[4] = p.nl; plan6[5] = p.nr;
+  flags3[0] = p.t_lhs; flags3[1] = p.t_rhs; flags3[2] = p.t_out;
+  return 0;
+}
+
+int orc_contraction_run(const int32_t *params, int nparams, uint32_t rank_l, const int64_t *dims_l,
+                        uint32_t rank_r, const int64_t *dims_r, const float *lhs, const float *rhs,
+                        double *out, int brute) {
+  orc_cinfo_t info;
+  if (orc_contraction_parse(params, nparams, rank_l, dims_l, rank_r, dims_r, &info)) return -1;
+  return brute ? orc_contraction_brute(&info, lhs, rhs, out) : orc_contraction_cpu_op(&info, lhs, rhs, out);
+}
+
+/* Raw permute entry for tests (same semantics as permute_t). */
+void orc_permute_raw(const int64_t *dims, const int32_t *ordering, int rank, float *buf, float *tmp) {
+  int ord[ORC_MAXRANK];
+  for (int r = 0; r < rank; ++r) ord[r] = ordering[r];
+  orc_permute(dims, ord, rank, buf, tmp);
+}

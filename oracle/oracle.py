@@ -1,0 +1,270 @@
+"""CPU oracle for the block kernels -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
+import this module, and only as the checker or the timed CPU baseline.  The
+product (``bbts_amd``) never imports it.
+
+It wraps ``oracle/libbbts_oracle.so`` (built from ``oracle/bbts_oracle.c`` by
+``oracle/Makefile``), a plain-C restatement of the reference's CPU ud
+functions; each C function cites the reference file:line it follows.
+
+Pinning status (what checks each oracle function; see tests/test_oracle.py):
+  dense size / header ....... size law of the reference's tests/test_builtin_dense.cc:27-38
+  dense add ................. closed form 3(row+col) of tests/test_builtin_ud.cc:46-73
+  local reduce .............. integer sums of integration_tests/test_reduce.cc:101-111; k=2 == add
+  broadcast ................. byte-exact copies (integration_tests/test_broadcast.cc:91-93)
+  gemm (FP64) ............... brute force, identity/permutation/rank-1/integer closed forms, numpy
+  contraction plan path ..... brute force einsum (contraction.h:131-202) and numpy.einsum on
+                              every plan-consistent mode order; GEMM special cases
+  permute ................... numpy.transpose on every permutation of rank <= 4
+  rng ....................... moments; the counter mix against its published test values
+Nothing is "parity unpinned".
+"""
+import ctypes
+import os
+import struct
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libbbts_oracle.so")
+_lib = None
+
+f32p = ctypes.POINTER(ctypes.c_float)
+f64p = ctypes.POINTER(ctypes.c_double)
+i64p = ctypes.POINTER(ctypes.c_int64)
+i32p = ctypes.POINTER(ctypes.c_int32)
+u8p = ctypes.POINTER(ctypes.c_uint8)
+
+META_BYTES = 260
+
+
+def build():
+    """Compile the oracle library (plain gcc through oracle/Makefile)."""
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_dense_size.restype = ctypes.c_size_t
+        L.orc_dense_size.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+        L.orc_dense_header.argtypes = [u8p, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32]
+        L.orc_cu_header.argtypes = [u8p, ctypes.c_int32, ctypes.c_uint32, i64p]
+        L.orc_cu_size.restype = ctypes.c_size_t
+        L.orc_cu_size.argtypes = [ctypes.c_uint32, i64p]
+        L.orc_mult_hint.restype = ctypes.c_uint64
+        L.orc_mult_hint.argtypes = [ctypes.c_uint32] * 3
+        L.orc_add_hint.restype = ctypes.c_uint64
+        L.orc_add_hint.argtypes = [ctypes.c_uint32] * 2
+        L.orc_rng_u64.restype = ctypes.c_uint64
+        L.orc_rng_u64.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+        L.orc_rng_uniform.argtypes = [f32p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint64,
+                                      ctypes.c_float, ctypes.c_float]
+        L.orc_rng_normal.argtypes = [f32p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint64,
+                                     ctypes.c_double, ctypes.c_double]
+        L.orc_dense_add.argtypes = [f32p, f32p, f32p, ctypes.c_uint32, ctypes.c_uint32]
+        L.orc_local_reduce.argtypes = [ctypes.POINTER(f32p), ctypes.c_int, f32p, ctypes.c_size_t]
+        L.orc_broadcast.argtypes = [u8p, ctypes.c_size_t, ctypes.POINTER(u8p), ctypes.c_int]
+        L.orc_relu.argtypes = [f32p, f32p, ctypes.c_size_t]
+        L.orc_gemm_f64.argtypes = [f32p, f32p, f64p, ctypes.c_uint32, ctypes.c_uint32,
+                                   ctypes.c_uint32, ctypes.c_int, ctypes.c_int]
+        L.orc_dgemm_rm.argtypes = [f64p, f64p, f64p] + [ctypes.c_uint32] * 6
+        L.orc_sgemm_provider.restype = ctypes.c_char_p
+        L.orc_dense_mult_f32.argtypes = [f32p, f32p, f32p, ctypes.c_uint32, ctypes.c_uint32,
+                                         ctypes.c_uint32, ctypes.c_int, ctypes.c_int]
+        L.orc_contraction_info.restype = ctypes.c_int
+        L.orc_contraction_info.argtypes = [i32p, ctypes.c_int, ctypes.c_uint32, i64p,
+                                           ctypes.c_uint32, i64p, i64p, i32p, i64p, i32p]
+        L.orc_contraction_run.restype = ctypes.c_int
+        L.orc_contraction_run.argtypes = [i32p, ctypes.c_int, ctypes.c_uint32, i64p,
+                                          ctypes.c_uint32, i64p, f32p, f32p, f64p, ctypes.c_int]
+        L.orc_permute_raw.argtypes = [i64p, i32p, ctypes.c_int, f32p, f32p]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+# ---------------------------------------------------------------- layout
+def dense_size(rows, cols):
+    return lib().orc_dense_size(rows, cols)
+
+
+def dense_header(rows, cols, fmt_id=0):
+    h = np.zeros(META_BYTES, dtype=np.uint8)
+    lib().orc_dense_header(_p(h, u8p), fmt_id, rows, cols)
+    return h
+
+
+def cu_header(dims, fmt_id=1):
+    d = np.zeros(4, dtype=np.int64)
+    d[:len(dims)] = dims
+    h = np.zeros(META_BYTES, dtype=np.uint8)
+    lib().orc_cu_header(_p(h, u8p), fmt_id, len(dims), _p(d, i64p))
+    return h
+
+
+def cu_size(dims):
+    d = np.zeros(4, dtype=np.int64)
+    d[:len(dims)] = dims
+    return lib().orc_cu_size(len(dims), _p(d, i64p))
+
+
+def mult_hint(a_rows, a_cols, b_cols):
+    return lib().orc_mult_hint(a_rows, a_cols, b_cols)
+
+
+def add_hint(rows, cols):
+    return lib().orc_add_hint(rows, cols)
+
+
+# ---------------------------------------------------------------- rng
+def rng_u64(seed, i):
+    return lib().orc_rng_u64(seed, i)
+
+
+def uniform(n, seed, left=-1.0, right=1.0, offset=0):
+    out = np.empty(n, dtype=np.float32)
+    lib().orc_rng_uniform(_p(out, f32p), n, seed, offset, left, right)
+    return out
+
+
+def normal(n, seed, mean=0.0, std=1.0, offset=0):
+    out = np.empty(n, dtype=np.float32)
+    lib().orc_rng_normal(_p(out, f32p), n, seed, offset, mean, std)
+    return out
+
+
+# ---------------------------------------------------------------- dense ops
+def dense_add(a, b):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    out = np.empty_like(a)
+    rows, cols = a.shape
+    lib().orc_dense_add(_p(a, f32p), _p(b, f32p), _p(out, f32p), rows, cols)
+    return out
+
+
+def local_reduce(inputs):
+    ins = [np.ascontiguousarray(x, dtype=np.float32) for x in inputs]
+    arr = (f32p * len(ins))(*[_p(x, f32p) for x in ins])
+    out = np.empty_like(ins[0])
+    lib().orc_local_reduce(arr, len(ins), _p(out, f32p), ins[0].size)
+    return out
+
+
+def broadcast(src_bytes, n):
+    src = np.ascontiguousarray(src_bytes, dtype=np.uint8)
+    dsts = [np.zeros_like(src) for _ in range(n)]
+    arr = (u8p * n)(*[_p(d, u8p) for d in dsts])
+    lib().orc_broadcast(_p(src, u8p), src.size, arr, n)
+    return dsts
+
+
+def relu(x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty_like(x)
+    lib().orc_relu(_p(x, f32p), _p(out, f32p), x.size)
+    return out
+
+
+def gemm_f64(a, b, trans_a=False, trans_b=False):
+    """FP64 oracle of the reference's dense matmul (row-major, lda=K, ldb=J)."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    I, K = a.shape
+    J = b.shape[1]
+    out = np.empty((I, J), dtype=np.float64)
+    lib().orc_gemm_f64(_p(a, f32p), _p(b, f32p), _p(out, f64p), I, J, K, int(trans_a), int(trans_b))
+    return out
+
+
+def dgemm(a, b):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    M, K = a.shape
+    N = b.shape[1]
+    out = np.empty((M, N), dtype=np.float64)
+    lib().orc_dgemm_rm(_p(a, f64p), _p(b, f64p), _p(out, f64p), M, N, K, K, N, N)
+    return out
+
+
+def sgemm_provider():
+    return lib().orc_sgemm_provider().decode()
+
+
+def dense_mult_f32(a, b, trans_a=False, trans_b=False, out=None):
+    """The reference's FP32 call (cblas_sgemm row-major) -- the CPU baseline."""
+    I, K = a.shape
+    J = b.shape[1]
+    if out is None:
+        out = np.empty((I, J), dtype=np.float32)
+    lib().orc_dense_mult_f32(_p(a, f32p), _p(b, f32p), _p(out, f32p), I, J, K,
+                             int(trans_a), int(trans_b))
+    return out
+
+
+# ---------------------------------------------------------------- contraction
+def contraction_params(m_lhs, m_rhs, m_out, alpha=1.0):
+    """Encode the reference's params list [rank_l, rank_r, rank_o, modes..., alpha]."""
+    words = [len(m_lhs), len(m_rhs), len(m_out)] + list(m_lhs) + list(m_rhs) + list(m_out)
+    words.append(struct.unpack("<i", struct.pack("<f", alpha))[0])
+    return np.array(words, dtype=np.int32)
+
+
+def contraction_info(params, dims_l, dims_r):
+    dl = np.zeros(4, np.int64); dl[:len(dims_l)] = dims_l
+    dr = np.zeros(4, np.int64); dr[:len(dims_r)] = dims_r
+    od = np.zeros(4, np.int64)
+    orank = np.zeros(1, np.int32)
+    plan = np.zeros(6, np.int64)
+    flags = np.zeros(3, np.int32)
+    rc = lib().orc_contraction_info(_p(params, i32p), len(params), len(dims_l), _p(dl, i64p),
+                                    len(dims_r), _p(dr, i64p), _p(od, i64p), _p(orank, i32p),
+                                    _p(plan, i64p), _p(flags, i32p))
+    if rc != 0:
+        raise ValueError("contraction parse/plan failed: %d" % rc)
+    return {"out_dims": tuple(int(x) for x in od[:orank[0]]),
+            "ni": int(plan[0]), "nj": int(plan[1]), "nk": int(plan[2]),
+            "nb": int(plan[3]), "nl": int(plan[4]), "nr": int(plan[5]),
+            "t_lhs": bool(flags[0]), "t_rhs": bool(flags[1]), "t_out": bool(flags[2])}
+
+
+def contraction(params, lhs, dims_l, rhs, dims_r, brute=False):
+    """Run the reference contraction on column-major flat FP32 inputs.
+
+    brute=False follows cpu_op (plan, permutes, l/r pre-sum, batched product);
+    brute=True follows reference() (every incident index tuple).  Both
+    accumulate in FP64; returns the flat FP64 output in the reference layout.
+    """
+    info = contraction_info(params, dims_l, dims_r)
+    n = int(np.prod(info["out_dims"])) if info["out_dims"] else 1
+    out = np.zeros(max(n, 1), dtype=np.float64)
+    dl = np.zeros(4, np.int64); dl[:len(dims_l)] = dims_l
+    dr = np.zeros(4, np.int64); dr[:len(dims_r)] = dims_r
+    lhs = np.ascontiguousarray(lhs, dtype=np.float32).reshape(-1)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float32).reshape(-1)
+    rc = lib().orc_contraction_run(_p(params, i32p), len(params), len(dims_l), _p(dl, i64p),
+                                   len(dims_r), _p(dr, i64p), _p(lhs, f32p), _p(rhs, f32p),
+                                   _p(out, f64p), int(brute))
+    if rc != 0:
+        raise ValueError("contraction failed: %d" % rc)
+    return out[:n], info["out_dims"]
+
+
+def permute(buf, dims, ordering):
+    """permute_t restated: reorder a column-major tensor so that mode r of the
+    input ends at position ordering[r]."""
+    b = np.ascontiguousarray(buf, dtype=np.float32).reshape(-1).copy()
+    tmp = np.empty_like(b)
+    d = np.array(list(dims) + [0] * (4 - len(dims)), dtype=np.int64)
+    o = np.array(list(ordering) + [0] * (4 - len(ordering)), dtype=np.int32)
+    lib().orc_permute_raw(_p(d, i64p), _p(o, i32p), len(dims), _p(b, f32p), _p(tmp, f32p))
+    return b

@@ -1,0 +1,117 @@
+"""Grouped persistent stream of uneven block GEMMs: every op of the stream
+against the oracle, under every schedule policy and every tile config, and
+the captured hipGraph replayed several times against the eager launch.
+
+Bar: BF16 operands <= 2e-6 relative Frobenius error vs the FP64 product of the
+BF16 values; FP32 operands with BF16x3 <= 4e-5, BF16x6 <= 2e-6.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _stream(torch, orc, shapes, dtype, seed=0):
+    ops = []
+    for q, (M, N, K) in enumerate(shapes):
+        a = orc.normal(M * K, seed + 2 * q).reshape(M, K)
+        b = orc.normal(K * N, seed + 2 * q + 1).reshape(K, N)
+        if dtype == 1:
+            ta = torch.from_numpy(a).to(torch.bfloat16).cuda()
+            tb = torch.from_numpy(b).to(torch.bfloat16).cuda()
+            a = ta.float().cpu().numpy()
+            b = tb.float().cpu().numpy()
+        else:
+            ta = torch.from_numpy(a).cuda()
+            tb = torch.from_numpy(b).cuda()
+        C = torch.full((M, N), float("nan"), dtype=torch.float32, device="cuda")
+        ops.append((M, N, K, a, b, ta, tb, C))
+    return ops
+
+
+def _check(orc, ops, tol):
+    for (M, N, K, a, b, ta, tb, C) in ops:
+        ref = orc.dgemm(a.astype(np.float64), b.astype(np.float64))
+        c = C.cpu().numpy().astype(np.float64)
+        err = np.linalg.norm(c - ref) / max(np.linalg.norm(ref), 1e-300)
+        assert err <= tol, (M, N, K, err)
+
+
+SHAPES = [(128, 128, 128), (64, 192, 320), (1000, 64, 70), (256, 640, 128), (192, 1024, 512), (33, 17, 9),
+          (512, 512, 1024), (1024, 256, 64), (64, 64, 4096), (2048, 1024, 256)]
+
+
+@pytest.mark.parametrize("policy", [0, 1, 100, 101, 102, 103, 104])
+def test_grouped_bf16_policies(ctx, orc, torch_cuda, policy):
+    torch = torch_cuda
+    from bbts_amd.device import Batch
+    print("grouped policy %d" % policy, flush=True)
+    ops = _stream(torch, orc, SHAPES, dtype=1, seed=policy)
+    b = Batch(ctx)
+    for (M, N, K, a, bb, ta, tb, C) in ops:
+        b.add(M, N, K, ta.data_ptr(), tb.data_ptr(), C.data_ptr(), dtype=1, precision=1)
+    b.plan(policy)
+    b.launch()
+    ctx.sync()
+    _check(orc, ops, 2e-6)
+    b.close()
+
+
+@pytest.mark.parametrize("prec,tol", [(2, 4e-5), (3, 2e-6)])
+def test_grouped_fp32_split(ctx, orc, torch_cuda, prec, tol):
+    torch = torch_cuda
+    from bbts_amd.device import Batch
+    ops = _stream(torch, orc, SHAPES[:6], dtype=0, seed=50)
+    b = Batch(ctx)
+    for (M, N, K, a, bb, ta, tb, C) in ops:
+        b.add(M, N, K, ta.data_ptr(), tb.data_ptr(), C.data_ptr(), dtype=0, precision=prec)
+    b.plan(1)
+    b.launch()
+    ctx.sync()
+    _check(orc, ops, tol)
+    b.close()
+
+
+def test_graph_replays_match_eager(ctx, orc, torch_cuda):
+    torch = torch_cuda
+    from bbts_amd.device import Batch
+    ops = _stream(torch, orc, SHAPES, dtype=1, seed=77)
+    b = Batch(ctx)
+    for (M, N, K, a, bb, ta, tb, C) in ops:
+        b.add(M, N, K, ta.data_ptr(), tb.data_ptr(), C.data_ptr(), dtype=1, precision=1)
+    b.plan(1)
+    b.launch()
+    ctx.sync()
+    eager = [C.clone() for (*_, C) in ops]
+    b.capture()
+    for rep in range(4):
+        for (*_, C) in ops:
+            C.fill_(float("nan"))
+        torch.cuda.synchronize()
+        b.replay()
+        ctx.sync()
+        for e, (*_, C) in zip(eager, ops):
+            assert torch.equal(e, C), rep
+        # eager work between replays
+        b.launch()
+        ctx.sync()
+    _check(orc, ops, 2e-6)
+    b.close()
+
+
+def test_tuner_fills_table(ctx, orc, torch_cuda):
+    torch = torch_cuda
+    from bbts_amd.device import Batch
+    ops = _stream(torch, orc, SHAPES[:4], dtype=1, seed=5)
+    b = Batch(ctx)
+    for (M, N, K, a, bb, ta, tb, C) in ops:
+        b.add(M, N, K, ta.data_ptr(), tb.data_ptr(), C.data_ptr(), dtype=1, precision=1)
+    t = b.tune(reps=2)
+    assert t.shape == (4, 5) and np.all(t > 0)
+    b.plan(2)
+    for i in range(4):
+        assert b.op_config(i) == int(np.argmin(t[i]))
+    b.launch()
+    ctx.sync()
+    _check(orc, ops, 2e-6)
+    b.close()

@@ -1,0 +1,67 @@
+"""Contraction parity (libraries/cutensor/contraction.h semantics) through the C ABI.
+
+Bar: relative Frobenius error vs the FP64 brute-force oracle (reference(),
+contraction.h:131-202) <= 2e-6 for FP32 and BF16x6, <= 4e-5 for BF16x3; the
+output cutensor meta (rank, dims in m_out order) is byte-exact.
+"""
+import struct
+
+import numpy as np
+import pytest
+
+from test_oracle import CONTRACTIONS, _case
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_raw(ctx, torch, orc, spec, seed, prec, alpha=1.0):
+    import ctypes
+    from bbts_amd import call
+    p, L, dl, R, dr = _case(orc, spec, seed, alpha)
+    info = orc.contraction_info(p, dl, dr)
+    n = int(np.prod(info["out_dims"])) if info["out_dims"] else 1
+    tL, tR = torch.from_numpy(L).cuda(), torch.from_numpy(R).cuda()
+    out = torch.full((max(n, 1),), float("nan"), dtype=torch.float32, device="cuda")
+    d_l = (ctypes.c_int64 * 4)(*(list(dl) + [0] * (4 - len(dl))))
+    d_r = (ctypes.c_int64 * 4)(*(list(dr) + [0] * (4 - len(dr))))
+    call("bbts_hip_contraction_raw", ctx.h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(p), len(dl), d_l,
+         tL.data_ptr(), len(dr), d_r, tR.data_ptr(), out.data_ptr(), prec, None)
+    ctx.sync()
+    ref, _ = orc.contraction(p, L, dl, R, dr, brute=True)
+    return out.cpu().numpy()[:n].astype(np.float64), ref
+
+
+@pytest.mark.parametrize("spec", CONTRACTIONS)
+def test_contraction_fp32(ctx, orc, torch_cuda, spec):
+    print("contraction", spec, flush=True)
+    got, ref = _run_raw(ctx, torch_cuda, orc, spec, 3, 0, alpha=0.5)
+    assert np.linalg.norm(got - ref) <= 2e-6 * max(np.linalg.norm(ref), 1e-30)
+
+
+@pytest.mark.parametrize("spec", [("ij", "jk", "ik"), ("abcd", "cdef", "abef"), ("ijl", "jk", "ik")])
+@pytest.mark.parametrize("prec,tol", [(2, 4e-5), (3, 2e-6)])
+def test_contraction_split(ctx, orc, torch_cuda, spec, prec, tol):
+    got, ref = _run_raw(ctx, torch_cuda, orc, spec, 4, prec)
+    assert np.linalg.norm(got - ref) <= tol * max(np.linalg.norm(ref), 1e-30)
+
+
+def test_contraction_blocks_meta(ctx, orc):
+    # abcd,cdef->abef at a reduced size, through cutensor-format blocks
+    a_, b_, c_, d_, e_, f_ = 12, 10, 16, 8, 9, 11
+    dl, dr = [a_, b_, c_, d_], [c_, d_, e_, f_]
+    p = orc.contraction_params([0, 1, 2, 3], [2, 3, 4, 5], [0, 1, 4, 5], 1.0)
+    L = orc.normal(int(np.prod(dl)), 1)
+    R = orc.normal(int(np.prod(dr)), 2)
+    lhs, rhs = ctx.cutensor(dl, L), ctx.cutensor(dr, R)
+    out = ctx.alloc(orc.cu_header([1]).tobytes(), 260 + 4 * a_ * b_ * e_ * f_)
+    ctx.contraction(lhs, rhs, out, p)
+    ctx.sync()
+    assert out.cu_shape() == (a_, b_, e_, f_)
+    hdr = out.download_bytes(260).tobytes()
+    assert struct.unpack_from("<iI4q", hdr, 0)[:2] == (1, 4)
+    assert struct.unpack_from("<4q", hdr, 12) == (a_, b_, e_, f_)
+    got = out.download_payload(a_ * b_ * e_ * f_).astype(np.float64)
+    ref, _ = orc.contraction(p, L, dl, R, dr, brute=False)
+    assert np.linalg.norm(got - ref) <= 2e-6 * np.linalg.norm(ref)
+    for t in (lhs, rhs, out):
+        t.free()

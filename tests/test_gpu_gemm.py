@@ -1,0 +1,144 @@
+"""Block GEMM parity through the C ABI: FP32 MFMA drop-in and the BF16 split forms.
+
+Bar (relative Frobenius error vs the FP64 oracle, written per test):
+  FP32 drop-in ........ <= 2e-6 for K <= 2048 (FP32 accumulation of N(0,1) products)
+  BF16x1 .............. <= 1e-2 (one BF16 rounding of each operand)
+  BF16x3 .............. <= 4e-5  (hi+lo split, 3 products)
+  BF16x6 .............. <= 2e-6  (hi+mid+lo split, 6 products: FP32 level)
+  BF16 operands ....... <= 2e-6  (products exact, FP32 accumulation only)
+Integer-valued operands are exact in every form (products and sums below 2^24).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP32 = 2e-6
+TOL = {0: TOL_FP32, 1: 1e-2, 2: 4e-5, 3: 2e-6}
+
+
+def relerr(c, ref):
+    return float(np.linalg.norm(c.astype(np.float64) - ref) / max(np.linalg.norm(ref), 1e-300))
+
+
+def _dense_mm(ctx, orc, I, J, K, ta=False, tb=False, precision=0, seed=0):
+    a = orc.normal(I * K, seed + 1).reshape(I, K)
+    b = orc.normal(K * J, seed + 2).reshape(K, J)
+    A, B = ctx.dense(I, K, a), ctx.dense(K, J, b)
+    out = ctx.alloc(orc.dense_header(1, 1).tobytes(), 260 + 4 * I * J)
+    ctx.matmul(A, B, out, ta, tb, precision)
+    ctx.sync()
+    c = out.download_payload(I * J).reshape(I, J)
+    shape = out.dense_shape()
+    for t in (A, B, out):
+        t.free()
+    return c, orc.gemm_f64(a, b, ta, tb), shape
+
+
+@pytest.mark.parametrize("I,J,K", [(1, 1, 1), (3, 5, 7), (31, 33, 35), (64, 64, 64), (127, 129, 130),
+                                   (128, 256, 512), (257, 65, 1000), (1024, 1024, 1024)])
+def test_fp32_dropin_shapes(ctx, orc, I, J, K):
+    print("fp32 %dx%dx%d" % (I, J, K), flush=True)
+    c, ref, shape = _dense_mm(ctx, orc, I, J, K)
+    assert shape == (I, J)  # out meta {a.rows, b.cols} (dense_matrix_mult.cc:85)
+    assert relerr(c, ref) <= TOL_FP32
+
+
+@pytest.mark.parametrize("ta,tb", [(True, False), (False, True), (True, True)])
+def test_fp32_transposes_square(ctx, orc, ta, tb):
+    c, ref, _ = _dense_mm(ctx, orc, 96, 96, 96, ta, tb)
+    assert relerr(c, ref) <= TOL_FP32
+
+
+def test_transpose_nonsquare_rejected(ctx):
+    from bbts_amd import BBTSError
+    A, B, out = ctx.dense(8, 16), ctx.dense(16, 8), ctx.dense(8, 8)
+    with pytest.raises(BBTSError) as e:
+        ctx.matmul(A, B, out, True, False)
+    assert e.value.code == 6
+    for t in (A, B, out):
+        t.free()
+
+
+def test_integer_valued_exact(ctx, orc):
+    rng = np.random.default_rng(0)
+    a = rng.integers(-8, 8, (200, 300)).astype(np.float32)
+    b = rng.integers(-8, 8, (300, 150)).astype(np.float32)
+    exact = a.astype(np.int64) @ b.astype(np.int64)
+    for prec in (0, 1, 2, 3):
+        A, B, out = ctx.dense(200, 300, a), ctx.dense(300, 150, b), ctx.dense(200, 150)
+        ctx.matmul(A, B, out, precision=prec)
+        ctx.sync()
+        assert np.array_equal(out.download_payload(200 * 150).reshape(200, 150), exact.astype(np.float32)), prec
+        for t in (A, B, out):
+            t.free()
+
+
+def test_identity_and_permutation(ctx, orc):
+    n = 100
+    a = orc.normal(n * n, 3).reshape(n, n)
+    for m in (np.eye(n, dtype=np.float32), np.eye(n, dtype=np.float32)[np.random.default_rng(1).permutation(n)]):
+        A, M, out = ctx.dense(n, n, a), ctx.dense(n, n, m), ctx.dense(n, n)
+        ctx.matmul(M, A, out)
+        ctx.sync()
+        assert np.array_equal(out.download_payload(n * n).reshape(n, n), m @ a)
+        for t in (A, M, out):
+            t.free()
+
+
+@pytest.mark.parametrize("prec", [1, 2, 3])
+@pytest.mark.parametrize("I,J,K", [(5, 7, 9), (130, 70, 300), (512, 512, 2048)])
+def test_split_bf16_error_levels(ctx, orc, prec, I, J, K):
+    print("split prec=%d %dx%dx%d" % (prec, I, J, K), flush=True)
+    c, ref, _ = _dense_mm(ctx, orc, I, J, K, precision=prec, seed=7)
+    assert relerr(c, ref) <= TOL[prec]
+
+
+def test_split_bf16_transposed(ctx, orc):
+    c, ref, _ = _dense_mm(ctx, orc, 128, 128, 128, True, True, precision=3, seed=9)
+    assert relerr(c, ref) <= TOL[3]
+
+
+def test_raw_gemm_bf16_operands(ctx, orc, torch_cuda):
+    torch = torch_cuda
+    from bbts_amd.device import Batch
+    M, N, K = 300, 200, 700
+    a = orc.normal(M * K, 11).reshape(M, K)
+    b = orc.normal(K * N, 12).reshape(K, N)
+    ta = torch.from_numpy(a).cuda()
+    tb = torch.from_numpy(b).cuda()
+    abf = torch.empty(M * K, dtype=torch.int16, device="cuda")
+    bbf = torch.empty(K * N, dtype=torch.int16, device="cuda")
+    ctx.to_bf16(ta, abf)
+    ctx.to_bf16(tb, bbf)
+    C = torch.zeros(M, N, dtype=torch.float32, device="cuda")
+    batch = Batch(ctx)
+    batch.add(M, N, K, abf.data_ptr(), bbf.data_ptr(), C.data_ptr(), dtype=1)
+    batch.plan(1)
+    batch.launch()
+    ctx.sync()
+    # the exact product of the BF16-rounded operands
+    a16 = torch.from_numpy(a).to(torch.bfloat16).float().double().numpy()
+    b16 = torch.from_numpy(b).to(torch.bfloat16).float().double().numpy()
+    ref = orc.dgemm(a16, b16)
+    assert relerr(C.cpu().numpy(), ref) <= 2e-6
+    batch.close()
+
+
+def test_residual_f64(ctx, orc, torch_cuda):
+    torch = torch_cuda
+    M, N, K = 200, 150, 333
+    a = orc.normal(M * K, 21).reshape(M, K)
+    b = orc.normal(K * N, 22).reshape(K, N)
+    c = (a.astype(np.float64) @ b.astype(np.float64)).astype(np.float32)
+    c[3, 4] += 1.0
+    out2 = torch.zeros(2, dtype=torch.float64, device="cuda")
+    A, B, C = (torch.from_numpy(x).cuda() for x in (a, b, c))
+    ctx.residual(M, N, K, A, K, 0, B, N, 0, C, N, out2)
+    ctx.sync()
+    ref = orc.gemm_f64(a, b)
+    se = float(((c.astype(np.float64) - ref) ** 2).sum())
+    sr = float((ref ** 2).sum())
+    got = out2.cpu().numpy()
+    assert abs(got[0] - se) <= 1e-9 * max(se, 1.0)
+    assert abs(got[1] - sr) <= 1e-12 * sr

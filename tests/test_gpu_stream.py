@@ -1,0 +1,154 @@
+"""Parity of the HBM-bound block kernels with the oracle, through the C ABI.
+
+Bar: bit-exact for add / reduce / ReLU / broadcast / uniform generation (the
+kernels perform the reference's IEEE operations in the reference's order),
+and byte-exact block headers.
+"""
+import struct
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+def test_add_closed_form_and_meta(ctx, orc):
+    # tests/test_builtin_ud.cc:46-73 on 100 x 200
+    r, c = np.meshgrid(np.arange(100), np.arange(200), indexing="ij")
+    a = ctx.dense(100, 200, (r + c).astype(np.float32))
+    b = ctx.dense(100, 200, (2 * (r + c)).astype(np.float32))
+    # the output's header says 7 x 7 on purpose: the ud rewrites the meta (dense_matrix_add.cc:76)
+    out = ctx.alloc(orc.dense_header(7, 7).tobytes(), 260 + 4 * 100 * 200)
+    ctx.add(a, b, out)
+    ctx.sync()
+    got = out.download_payload(100 * 200).reshape(100, 200)
+    assert np.array_equal(got, orc.dense_add((r + c).astype(np.float32), (2 * (r + c)).astype(np.float32)))
+    assert out.dense_shape() == (100, 200)
+    hdr = out.download_bytes(260)
+    assert struct.unpack_from("<iII", hdr.tobytes(), 0) == (0, 100, 200)
+    for x in (a, b, out):
+        x.free()
+
+
+@pytest.mark.parametrize("rows,cols", [(1, 1), (3, 5), (17, 33), (1024, 1024), (999, 1001)])
+def test_add_random_bitexact(ctx, orc, rows, cols):
+    _flush("add %dx%d" % (rows, cols))
+    x = orc.normal(rows * cols, 1).reshape(rows, cols)
+    y = orc.normal(rows * cols, 2).reshape(rows, cols)
+    a, b, out = ctx.dense(rows, cols, x), ctx.dense(rows, cols, y), ctx.dense(rows, cols)
+    ctx.add(a, b, out)
+    ctx.sync()
+    assert np.array_equal(out.download_payload(rows * cols).reshape(rows, cols), orc.dense_add(x, y))
+    # in-place: the output aliases input a (inputInplace {0,1}, dense_matrix_add.cc:15)
+    ctx.add(a, b, a)
+    ctx.sync()
+    assert np.array_equal(a.download_payload(rows * cols).reshape(rows, cols), orc.dense_add(x, y))
+    for t in (a, b, out):
+        t.free()
+
+
+def test_add_shape_mismatch_is_an_error(ctx):
+    from bbts_amd import BBTSError
+    a, b, out = ctx.dense(4, 4), ctx.dense(4, 5), ctx.dense(4, 4)
+    with pytest.raises(BBTSError) as e:
+        ctx.add(a, b, out)
+    assert e.value.code == 2
+    for t in (a, b, out):
+        t.free()
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 8, 31, 32, 33, 40])
+def test_reduce_left_fold_bitexact(ctx, orc, k):
+    _flush("reduce k=%d" % k)
+    rows, cols = 257, 129
+    xs = [orc.normal(rows * cols, 10 + q) * (10.0 ** (q % 5)) for q in range(k)]
+    blocks = [ctx.dense(rows, cols, x) for x in xs]
+    out = ctx.dense(rows, cols)
+    ctx.reduce(blocks, out)
+    ctx.sync()
+    assert np.array_equal(out.download_payload(rows * cols), orc.local_reduce(xs))
+    for t in blocks + [out]:
+        t.free()
+
+
+def test_reduce_integer_sums(ctx):
+    # integration_tests/test_reduce.cc:101-111 (sum over ranks of 1 + rank + i), here k = 8 blocks
+    n = 2048
+    xs = [(1 + rank + np.arange(n)).astype(np.float32) for rank in range(8)]
+    blocks = [ctx.dense(32, 64, x) for x in xs]
+    out = ctx.dense(32, 64)
+    ctx.reduce(blocks, out)
+    ctx.sync()
+    assert np.array_equal(out.download_payload(n), sum(xs).astype(np.float32))
+    for t in blocks + [out]:
+        t.free()
+
+
+@pytest.mark.parametrize("rows,cols,n", [(1, 3, 1), (64, 64, 5), (2048, 2048, 32), (33, 17, 70)])
+def test_broadcast_byte_exact(ctx, orc, rows, cols, n):
+    _flush("broadcast %dx%d -> %d" % (rows, cols, n))
+    x = orc.uniform(rows * cols, 3)
+    src = ctx.dense(rows, cols, x)
+    dsts = [ctx.alloc(orc.dense_header(1, 1).tobytes(), 260 + 4 * rows * cols) for _ in range(n)]
+    ctx.broadcast(src, dsts)
+    ctx.sync()
+    ref = src.download_bytes()
+    for d in dsts:
+        assert np.array_equal(d.download_bytes(), ref)
+        assert d.dense_shape() == (rows, cols)
+    for t in dsts + [src]:
+        t.free()
+
+
+def test_relu_exact(ctx, orc):
+    x = orc.normal(300 * 301, 4)
+    a, out = ctx.dense(300, 301, x), ctx.dense(300, 301)
+    ctx.relu(a, out)
+    ctx.sync()
+    assert np.array_equal(out.download_payload(300 * 301), orc.relu(x))
+    a.free()
+    out.free()
+
+
+def test_uniform_generator_bitexact(ctx, orc, torch_cuda):
+    torch = torch_cuda
+    n = 1 << 20
+    t = torch.empty(n, dtype=torch.float32, device="cuda")
+    ctx.gen_uniform(t, seed=42, left=-1.0, right=1.0, offset=5)
+    torch.cuda.synchronize()
+    assert np.array_equal(t.cpu().numpy(), orc.uniform(n, 42, -1.0, 1.0, offset=5))
+    blk = ctx.alloc(orc.dense_header(0, 0).tobytes(), 260 + 4 * 64 * 48)
+    ctx.uniform(blk, 64, 48, 0.0, 1.0, 11)
+    ctx.sync()
+    assert blk.dense_shape() == (64, 48)
+    assert np.array_equal(blk.download_payload(64 * 48), orc.uniform(64 * 48, 11, 0.0, 1.0))
+    blk.free()
+
+
+def test_normal_generator_matches(ctx, orc, torch_cuda):
+    torch = torch_cuda
+    n = 1 << 18
+    t = torch.empty(n, dtype=torch.float32, device="cuda")
+    ctx.gen_normal(t, seed=5, mean=0.0, std=1.0)
+    torch.cuda.synchronize()
+    ref = orc.normal(n, 5)
+    got = t.cpu().numpy()
+    # device log/cos vs host libm: equal after rounding to FP32 except rare last-bit cases
+    assert np.max(np.abs(got - ref)) <= 4 * np.finfo(np.float32).eps * np.max(np.abs(ref))
+    assert np.mean(got == ref) > 0.999
+
+
+def test_pool_reuses_memory(ctx):
+    a = ctx.dense(512, 512)
+    in_use0, cached0 = ctx.pool_stats()
+    a.free()
+    in_use1, cached1 = ctx.pool_stats()
+    assert cached1 > cached0 and in_use1 < in_use0
+    b = ctx.dense(512, 512)
+    in_use2, cached2 = ctx.pool_stats()
+    assert cached2 == cached1 - (in_use0 - in_use1)
+    b.free()

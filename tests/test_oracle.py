@@ -1,0 +1,253 @@
+"""Pins the CPU oracle (oracle/bbts_oracle.c) before anything is checked against it.
+
+Each check is chosen so that a plausible slip in the restatement (a dropped
+term, a swapped index, a transposed operand, a wrong fold order) fails:
+closed forms from the reference's own tests, brute force on tiny inputs,
+numpy in FP64, and algebraic identities.
+"""
+import itertools
+import struct
+
+import numpy as np
+import pytest
+
+
+# ---------------------------------------------------------------- layout
+def test_dense_size_law(orc):
+    # tests/test_builtin_dense.cc:27-38: get_tensor_size == 260 + 4*r*c for r, c < 100
+    for r in range(0, 100, 7):
+        for c in range(0, 100, 11):
+            assert orc.dense_size(r, c) == 260 + 4 * r * c
+
+
+def test_dense_header_bytes(orc):
+    h = orc.dense_header(1024, 2048, fmt_id=0)
+    assert len(h) == 260
+    assert struct.unpack_from("<iII", bytes(h), 0) == (0, 1024, 2048)
+    assert not h[12:].any()
+
+
+def test_cu_header_bytes(orc):
+    h = bytes(orc.cu_header([96, 96, 128, 128], fmt_id=1))
+    # cu_shape_t {u32 rank; i64 dims[4]} at the blob (+4): rank at +4, dims at +12
+    assert struct.unpack_from("<iI", h, 0) == (1, 4)
+    assert struct.unpack_from("<4q", h, 12) == (96, 96, 128, 128)
+    assert orc.cu_size([96, 96, 128, 128]) == 260 + 4 * 96 * 96 * 128 * 128
+
+
+def test_hint_overflow_quirk(orc):
+    # dense_matrix_mult.cc:32 computes rows*cols*cols in uint32: 2048^3 wraps to 0
+    assert orc.mult_hint(2048, 2048, 2048) == 0
+    assert orc.mult_hint(1024, 1024, 1024) == 2 ** 30
+    assert orc.mult_hint(1000, 1000, 1000) == 10 ** 9
+
+
+# ---------------------------------------------------------------- rng
+def test_splitmix_published_values(orc):
+    # splitmix64 reference output for seed 1234567 (first five draws)
+    expect = [6457827717110365317, 3203168211198807973, 9817491932198370423,
+              4593380528125082431, 16408922859458223821]
+    assert [orc.rng_u64(1234567, i) for i in range(5)] == expect
+
+
+def test_uniform_moments(orc):
+    x = orc.uniform(1 << 20, seed=7, left=-1.0, right=1.0)
+    assert x.min() >= -1.0 and x.max() < 1.0
+    assert abs(x.mean()) < 3e-3
+    assert abs(x.var() - 1.0 / 3.0) < 3e-3
+
+
+def test_normal_moments(orc):
+    x = orc.normal(1 << 20, seed=9).astype(np.float64)
+    assert abs(x.mean()) < 5e-3
+    assert abs(x.std() - 1.0) < 5e-3
+    assert abs(((x - x.mean()) ** 3).mean()) < 2e-2          # skewness ~ 0
+    assert abs((x ** 4).mean() / x.var() ** 2 - 3.0) < 5e-2  # kurtosis ~ 3
+
+
+# ---------------------------------------------------------------- add / reduce / broadcast / relu
+def test_add_closed_form(orc):
+    # tests/test_builtin_ud.cc:46-73: a = row+col, b = 2(row+col) -> 3(row+col), on 100x200
+    r, c = np.meshgrid(np.arange(100), np.arange(200), indexing="ij")
+    a = (r + c).astype(np.float32)
+    b = (2 * (r + c)).astype(np.float32)
+    out = orc.dense_add(a, b)
+    assert np.array_equal(out, (3 * (r + c)).astype(np.float32))
+
+
+def test_reduce_integer_sums(orc):
+    # integration_tests/test_reduce.cc:101-111: sums of (1 + rank + i) are exact
+    n = 1000
+    ins = [(1 + rank + np.arange(n)).astype(np.float32) for rank in range(0, 16, 2)]
+    out = orc.local_reduce(ins)
+    assert np.array_equal(out, sum(ins).astype(np.float32))
+
+
+def test_reduce_left_fold_order(orc):
+    # a fold order change is visible: (big + (-big)) + small != big + ((-big) + small) in FP32
+    big = np.full(4, 1e8, np.float32)
+    small = np.full(4, 1.0, np.float32)
+    out = orc.local_reduce([big, -big, small])
+    assert np.all(out == 1.0)
+    out2 = orc.local_reduce([small, big, -big])
+    assert np.all(out2 == 0.0)  # (1 + 1e8) rounds to 1e8, then - 1e8
+    # k=2 is the add ud bit for bit
+    x = orc.normal(4096, 3)
+    y = orc.normal(4096, 4)
+    assert np.array_equal(orc.local_reduce([x, y]), orc.dense_add(x.reshape(64, 64), y.reshape(64, 64)).reshape(-1))
+
+
+def test_broadcast_bytes(orc):
+    src = np.random.default_rng(0).integers(0, 256, 5000, dtype=np.uint8)
+    for d in orc.broadcast(src, 5):
+        assert np.array_equal(d, src)
+
+
+def test_relu(orc):
+    x = np.array([-2.0, -0.0, 0.0, 1e-30, 3.5], np.float32)
+    assert np.array_equal(orc.relu(x), np.array([0, 0, 0, 1e-30, 3.5], np.float32))
+
+
+# ---------------------------------------------------------------- gemm
+def _brute(a, b):
+    I, K = a.shape
+    J = b.shape[1]
+    out = np.zeros((I, J))
+    for i in range(I):
+        for j in range(J):
+            s = 0.0
+            for k in range(K):
+                s += float(a[i, k]) * float(b[k, j])
+            out[i, j] = s
+    return out
+
+
+@pytest.mark.parametrize("I,J,K", [(1, 1, 1), (3, 5, 7), (8, 1, 9), (5, 6, 1)])
+def test_gemm_brute_force(orc, I, J, K):
+    a = orc.normal(I * K, 1).reshape(I, K)
+    b = orc.normal(K * J, 2).reshape(K, J)
+    assert np.allclose(orc.gemm_f64(a, b), _brute(a, b), rtol=0, atol=1e-12)
+
+
+def test_gemm_closed_forms(orc):
+    n = 37
+    a = orc.normal(n * n, 5).reshape(n, n)
+    eye = np.eye(n, dtype=np.float32)
+    assert np.array_equal(orc.gemm_f64(a, eye), a.astype(np.float64))
+    assert np.array_equal(orc.gemm_f64(eye, a), a.astype(np.float64))
+    perm = np.eye(n, dtype=np.float32)[np.random.default_rng(1).permutation(n)]
+    assert np.array_equal(orc.gemm_f64(perm, a), (perm.astype(np.float64) @ a))
+    u = orc.normal(n, 6).reshape(n, 1)
+    v = orc.normal(n, 7).reshape(1, n)
+    assert np.allclose(orc.gemm_f64(u, v), u.astype(np.float64) * v.astype(np.float64), rtol=0, atol=0)
+    ia = np.random.default_rng(2).integers(-8, 8, (n, 29)).astype(np.float32)
+    ib = np.random.default_rng(3).integers(-8, 8, (29, n)).astype(np.float32)
+    assert np.array_equal(orc.gemm_f64(ia, ib), ia.astype(np.int64) @ ib.astype(np.int64))
+
+
+def test_gemm_transposes_square(orc):
+    n = 23
+    a = orc.normal(n * n, 8).reshape(n, n)
+    b = orc.normal(n * n, 9).reshape(n, n)
+    a64, b64 = a.astype(np.float64), b.astype(np.float64)
+    assert np.allclose(orc.gemm_f64(a, b, True, False), a64.T @ b64, atol=1e-12)
+    assert np.allclose(orc.gemm_f64(a, b, False, True), a64 @ b64.T, atol=1e-12)
+    assert np.allclose(orc.gemm_f64(a, b, True, True), a64.T @ b64.T, atol=1e-12)
+
+
+def test_dense_mult_f32_matches_f64(orc):
+    a = orc.normal(64 * 48, 10).reshape(64, 48)
+    b = orc.normal(48 * 80, 11).reshape(48, 80)
+    c = orc.dense_mult_f32(a, b)
+    ref = orc.gemm_f64(a, b)
+    assert np.linalg.norm(c - ref) / np.linalg.norm(ref) < 1e-6
+
+
+def test_dgemm(orc):
+    a = np.random.default_rng(4).standard_normal((9, 7))
+    b = np.random.default_rng(5).standard_normal((7, 11))
+    assert np.allclose(orc.dgemm(a, b), a @ b, rtol=1e-14, atol=1e-14)
+
+
+# ---------------------------------------------------------------- permute / contraction
+@pytest.mark.parametrize("dims", [(2, 3), (2, 3, 4), (2, 3, 4, 5)])
+def test_permute_every_order(orc, dims):
+    rank = len(dims)
+    x = np.arange(int(np.prod(dims)), dtype=np.float32)
+    X = x.reshape(dims[::-1]).transpose()  # column-major view
+    for ordering in itertools.permutations(range(rank)):
+        y = orc.permute(x, dims, ordering)
+        inv = [0] * rank
+        for r, p in enumerate(ordering):
+            inv[p] = r
+        Y = np.transpose(X, inv)
+        assert np.array_equal(Y.transpose().reshape(-1), y), ordering
+
+
+def _einsum_colmajor(spec_l, spec_r, spec_o, L, dl, R, dr, alpha):
+    Lt = L.astype(np.float64).reshape(dl[::-1]).transpose()
+    Rt = R.astype(np.float64).reshape(dr[::-1]).transpose()
+    O = alpha * np.einsum("%s,%s->%s" % (spec_l, spec_r, spec_o), Lt, Rt)
+    return O.transpose().reshape(-1)
+
+
+CONTRACTIONS = [
+    # (lhs, rhs, out) mode strings; letters are incident modes
+    ("ij", "jk", "ik"), ("ij", "jk", "ki"), ("ji", "jk", "ik"), ("ij", "kj", "ik"), ("ji", "kj", "ki"),
+    ("ijb", "jkb", "ikb"), ("ijb", "jkb", "kib"), ("abcd", "cdef", "abef"), ("abc", "bcd", "ad"),
+    ("ijl", "jk", "ik"), ("ij", "jkr", "ik"), ("ijlb", "jkbr", "ikb"),
+    ("bij", "jkb", "ikb"), ("ijb", "jkb", "bik"), ("iaj", "jb", "abi"),  # orders the plan path does not cover
+    ("i", "i", ""), ("i", "j", "ij"),
+]
+
+
+def _case(orc, spec, seed, alpha=1.0):
+    spec_l, spec_r, spec_o = spec
+    letters = sorted(set(spec_l + spec_r))
+    ids = {c: n for n, c in enumerate(letters)}
+    sizes = {c: 2 + (n * 3 + seed) % 4 for n, c in enumerate(letters)}
+    dl = [sizes[c] for c in spec_l]
+    dr = [sizes[c] for c in spec_r]
+    p = orc.contraction_params([ids[c] for c in spec_l], [ids[c] for c in spec_r], [ids[c] for c in spec_o], alpha)
+    L = orc.normal(int(np.prod(dl)), seed)
+    R = orc.normal(int(np.prod(dr)), seed + 100)
+    return p, L, dl, R, dr
+
+
+@pytest.mark.parametrize("spec", CONTRACTIONS)
+def test_contraction_brute_equals_einsum(orc, spec):
+    p, L, dl, R, dr = _case(orc, spec, 3, alpha=0.75)
+    out, _ = orc.contraction(p, L, dl, R, dr, brute=True)
+    ref = _einsum_colmajor(*spec, L, dl, R, dr, 0.75)
+    assert np.allclose(out, ref, rtol=1e-12, atol=1e-12)
+
+
+PLAN_CONSISTENT = CONTRACTIONS[:12]
+
+
+@pytest.mark.parametrize("spec", PLAN_CONSISTENT)
+def test_contraction_plan_path_equals_brute(orc, spec):
+    # cpu_op (permutes + l/r pre-sum + batched product) agrees with reference() on these orders
+    p, L, dl, R, dr = _case(orc, spec, 5, alpha=1.5)
+    a, _ = orc.contraction(p, L, dl, R, dr, brute=False)
+    b, _ = orc.contraction(p, L, dl, R, dr, brute=True)
+    assert np.allclose(a, b, rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("spec", [("bij", "jkb", "ikb"), ("ijb", "jkb", "bik")])
+def test_contraction_plan_path_defect(orc, spec):
+    # On these mode orders the reference's cpu_op lays data out differently from its own
+    # reference() checker (permute_t applies the inverse permutation for 3-cycles; the
+    # output permute is always the identity).  The build follows reference().
+    p, L, dl, R, dr = _case(orc, spec, 5)
+    a, _ = orc.contraction(p, L, dl, R, dr, brute=False)
+    b, _ = orc.contraction(p, L, dl, R, dr, brute=True)
+    assert not np.allclose(a, b, rtol=1e-3, atol=1e-3)
+
+
+def test_contraction_plan_groups(orc):
+    p = orc.contraction_params([0, 1, 2, 3], [2, 3, 4, 5], [0, 1, 4, 5])
+    info = orc.contraction_info(p, [96, 96, 128, 128], [128, 128, 96, 96])
+    assert info["out_dims"] == (96, 96, 96, 96)
+    assert (info["ni"], info["nj"], info["nk"], info["nb"]) == (9216, 16384, 9216, 1)
+    assert not (info["t_lhs"] or info["t_rhs"] or info["t_out"])
