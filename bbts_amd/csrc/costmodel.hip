@@ -21,7 +21,8 @@
 namespace bbts_hip {
 namespace {
 
-constexpr int kNF = 16;
+constexpr int kNF = 18;
+constexpr int kMaxCfgs = 6;
 constexpr int kThreads = 256;
 
 // Y[s, o] = act(sum_i X[s, i] * W[o, i] + b[o]);  Z keeps the pre-activation.
@@ -250,10 +251,11 @@ int bbts_hip_costmodel_features(int M, int N, int K, int dtype, int precision, i
   f[5] = (float)(lg(2.0 * M * N * (double)K) / 40.0);
   f[6] = (float)(slices / 3.0);
   f[7] = (float)dtype;
-  for (int c = 0; c < 5; ++c) f[8 + c] = (c == cfg) ? 1.0f : 0.0f;
-  f[13] = (float)(lg(bm) / 8.0);
-  f[14] = (float)(lg(bn) / 8.0);
-  f[15] = (float)(lg(tiles * bm * bn * (double)K * 2.0) / 40.0);  // padded flops
+  for (int c = 0; c < kMaxCfgs; ++c) f[8 + c] = (c == cfg) ? 1.0f : 0.0f;
+  f[14] = (float)(lg(bm) / 8.0);
+  f[15] = (float)(lg(bn) / 8.0);
+  f[16] = (float)(lg(tiles * bm * bn * (double)K * 2.0) / 40.0);  // padded flops
+  f[17] = (float)(lg(tiles / 256.0 + 1.0) / 4.0);                  // waves of tiles over 256 CUs
   return BBTS_OK;
 }
 

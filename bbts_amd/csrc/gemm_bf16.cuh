@@ -130,7 +130,9 @@ __device__ __forceinline__ void tile(const op_view &op, int m0, int n0, uint16_t
     for (int r = 0; r < B_PER; ++r) {
       int q = tid + kThreads * r;
       if (q < B_ITEMS) {
-        int kp = q / (BN / EPV), ng = q % (BN / EPV);
+        // k-pair fastest across lanes: the transposing 32-bit LDS stores of one
+        // instruction then spread over the 16 banks of a row half (4-way, not 16-way)
+        int kp = q % (BK / 2), ng = q / (BK / 2);
         int gk = k0 + kp * 2, gn = n0 + ng * EPV;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -177,7 +179,7 @@ __device__ __forceinline__ void tile(const op_view &op, int m0, int n0, uint16_t
     for (int r = 0; r < B_PER; ++r) {
       int q = tid + kThreads * r;
       if (q < B_ITEMS) {
-        int kp = q / (BN / EPV), ng = q % (BN / EPV);
+        int kp = q % (BK / 2), ng = q / (BK / 2);
         int k = kp * 2, n = ng * EPV;
         if constexpr (EPV == 8) {
           const uint16_t *h0 = reinterpret_cast<const uint16_t *>(&rb[r][0]);

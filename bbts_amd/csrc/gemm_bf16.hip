@@ -82,7 +82,7 @@ __global__ __launch_bounds__(256) void gemm_grouped_kernel(const gop *__restrict
 struct cfg_shape {
   int bm, bn;
 };
-constexpr cfg_shape kCfgs[] = {{256, 128}, {128, 128}, {128, 64}, {64, 64}, {64, 128}};
+constexpr cfg_shape kCfgs[] = {{256, 128}, {128, 128}, {128, 64}, {64, 64}, {64, 128}, {256, 256}};
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
 // variant: 0 = BF16 operands (S=T=1); 1,2,3 = FP32 operands split into 1,2,3 slices (T=S)
@@ -120,7 +120,8 @@ single_fn get_single(int cfg, int variant) {
     case 1: return cfg_kernels<128, 128>::single(variant);
     case 2: return cfg_kernels<128, 64>::single(variant);
     case 3: return cfg_kernels<64, 64>::single(variant);
-    default: return cfg_kernels<64, 128>::single(variant);
+    case 4: return cfg_kernels<64, 128>::single(variant);
+    default: return cfg_kernels<256, 256>::single(variant);
   }
 }
 grouped_fn get_grouped(int cfg, int variant) {
@@ -129,7 +130,8 @@ grouped_fn get_grouped(int cfg, int variant) {
     case 1: return cfg_kernels<128, 128>::grouped(variant);
     case 2: return cfg_kernels<128, 64>::grouped(variant);
     case 3: return cfg_kernels<64, 64>::grouped(variant);
-    default: return cfg_kernels<64, 128>::grouped(variant);
+    case 4: return cfg_kernels<64, 128>::grouped(variant);
+    default: return cfg_kernels<256, 256>::grouped(variant);
   }
 }
 
@@ -150,7 +152,7 @@ int occupancy(int cfg, int variant) {
   int b = smem_bytes(cfg, variant) + 64;
   int by_lds = (160 * 1024) / b;
   // waves per SIMD the register budget allows (4 waves per workgroup, one per SIMD)
-  static const int kByRegs[] = {1, 2, 3, 4, 3};
+  static const int kByRegs[] = {1, 2, 3, 4, 3, 1};
   int by_waves = kByRegs[cfg];
   int o = by_lds < by_waves ? by_lds : by_waves;
   return o < 1 ? 1 : o;

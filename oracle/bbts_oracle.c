@@ -216,6 +216,10 @@ void orc_dgemm_rm(const double *A, const double *B, double *C, uint32_t M, uint3
 typedef void (*orc_cblas_sgemm_t)(int, int, int, int, int, int, float, const float *, int,
                                   const float *, int, float, float *, int);
 static orc_cblas_sgemm_t orc_sgemm_fn = 0;
+typedef void (*orc_set_threads_t)(int);
+typedef int (*orc_get_threads_t)(void);
+static orc_set_threads_t orc_set_threads_fn = 0;
+static orc_get_threads_t orc_get_threads_fn = 0;
 static int orc_sgemm_resolved = 0;
 static char orc_sgemm_name[256] = "loops";
 
@@ -231,6 +235,10 @@ static void orc_resolve_sgemm(void) {
     void *f = dlsym(h, "cblas_sgemm");
     if (f) {
       orc_sgemm_fn = (orc_cblas_sgemm_t)f;
+      orc_set_threads_fn = (orc_set_threads_t)dlsym(h, "MKL_Set_Num_Threads");
+      if (!orc_set_threads_fn) orc_set_threads_fn = (orc_set_threads_t)dlsym(h, "openblas_set_num_threads");
+      orc_get_threads_fn = (orc_get_threads_t)dlsym(h, "MKL_Get_Max_Threads");
+      if (!orc_get_threads_fn) orc_get_threads_fn = (orc_get_threads_t)dlsym(h, "openblas_get_num_threads");
       strncpy(orc_sgemm_name, cands[i], sizeof(orc_sgemm_name) - 1);
       return;
     }
@@ -241,6 +249,13 @@ static void orc_resolve_sgemm(void) {
 const char *orc_sgemm_provider(void) {
   orc_resolve_sgemm();
   return orc_sgemm_name;
+}
+
+/* Thread count of the BLAS provider (set if n > 0); returns the count in use, or 1 for loops. */
+int orc_sgemm_threads(int n) {
+  orc_resolve_sgemm();
+  if (n > 0 && orc_set_threads_fn) orc_set_threads_fn(n);
+  return orc_get_threads_fn ? orc_get_threads_fn() : 1;
 }
 
 /* The reference's FP32 call, verbatim in meaning (row-major, lda=K, ldb=J, ldc=J). */

@@ -75,6 +75,8 @@ def lib():
                                    ctypes.c_uint32, ctypes.c_int, ctypes.c_int]
         L.orc_dgemm_rm.argtypes = [f64p, f64p, f64p] + [ctypes.c_uint32] * 6
         L.orc_sgemm_provider.restype = ctypes.c_char_p
+        L.orc_sgemm_threads.restype = ctypes.c_int
+        L.orc_sgemm_threads.argtypes = [ctypes.c_int]
         L.orc_dense_mult_f32.argtypes = [f32p, f32p, f32p, ctypes.c_uint32, ctypes.c_uint32,
                                          ctypes.c_uint32, ctypes.c_int, ctypes.c_int]
         L.orc_contraction_info.restype = ctypes.c_int
@@ -198,6 +200,11 @@ def dgemm(a, b):
 
 def sgemm_provider():
     return lib().orc_sgemm_provider().decode()
+
+
+def sgemm_threads(n=0):
+    """Set (n > 0) and return the BLAS provider's thread count."""
+    return lib().orc_sgemm_threads(n)
 
 
 def dense_mult_f32(a, b, trans_a=False, trans_b=False, out=None):

@@ -47,16 +47,16 @@ def test_library_loads_and_binds():
     assert L.bbts_hip_dense_size(1024, 1024) == 260 + 4 * 1024 * 1024
     assert L.bbts_hip_strerror(2) == b"shape mismatch"
     assert L.bbts_hip_num_tile_configs() >= 4
-    assert L.bbts_hip_costmodel_num_features() == 16
+    assert L.bbts_hip_costmodel_num_features() == 18
 
 
 def test_costmodel_features_are_host_side():
     _ensure_built()
     from bbts_amd.device import CostModel
     f = CostModel.features(1024, 512, 256, 1, 0, 1)
-    assert f.shape == (16,)
+    assert f.shape == (18,)
     assert abs(f[0] - 10 / 12) < 1e-6 and abs(f[1] - 9 / 12) < 1e-6 and abs(f[2] - 8 / 12) < 1e-6
-    assert f[8 + 1] == 1.0 and f[8:13].sum() == 1.0
+    assert f[8 + 1] == 1.0 and f[8:14].sum() == 1.0
 
 
 def test_init_without_gpu_fails_loudly():

@@ -41,7 +41,7 @@ SHAPES = [(128, 128, 128), (64, 192, 320), (1000, 64, 70), (256, 640, 128), (192
           (512, 512, 1024), (1024, 256, 64), (64, 64, 4096), (2048, 1024, 256)]
 
 
-@pytest.mark.parametrize("policy", [0, 1, 100, 101, 102, 103, 104])
+@pytest.mark.parametrize("policy", [0, 1, 100, 101, 102, 103, 104, 105])
 def test_grouped_bf16_policies(ctx, orc, torch_cuda, policy):
     torch = torch_cuda
     from bbts_amd.device import Batch
@@ -107,7 +107,7 @@ def test_tuner_fills_table(ctx, orc, torch_cuda):
     for (M, N, K, a, bb, ta, tb, C) in ops:
         b.add(M, N, K, ta.data_ptr(), tb.data_ptr(), C.data_ptr(), dtype=1, precision=1)
     t = b.tune(reps=2)
-    assert t.shape == (4, 5) and np.all(t > 0)
+    assert t.shape == (4, 6) and np.all(t > 0)
     b.plan(2)
     for i in range(4):
         assert b.op_config(i) == int(np.argmin(t[i]))
