@@ -111,6 +111,7 @@ _SIGS = {
     "bbts_hip_costmodel_features": [ctypes.c_int] * 6 + [_vp],
     "bbts_hip_costmodel_train": [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _f, ctypes.c_int, ctypes.POINTER(_f), _vp],
     "bbts_hip_costmodel_predict": [_vp, _vp, ctypes.c_int, _vp, _vp],
+    "bbts_hip_costmodel_predict_host": [_vp, _vp, ctypes.c_int, _vp],
     "bbts_hip_costmodel_grad": [_vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp],
     "bbts_hip_costmodel_num_params": [_vp],
     "bbts_hip_costmodel_get_params": [_vp, _vp, ctypes.c_int],

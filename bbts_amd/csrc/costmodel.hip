@@ -346,6 +346,15 @@ int bbts_hip_costmodel_predict(bbts_costmodel *cm, const float *X, int n, float 
   return BBTS_OK;
 }
 
+int bbts_hip_costmodel_predict_host(bbts_costmodel *cm, const float *X_host, int n, float *y_host) {
+  BBTS_REQUIRE(cm && X_host && y_host && n >= 0, BBTS_E_INVALID, "bad predict arguments");
+  std::vector<float> feats(X_host, X_host + (size_t)n * kNF), out;
+  int rc = costmodel_predict_host(cm, feats, n, out);
+  if (rc) return rc;
+  std::memcpy(y_host, out.data(), sizeof(float) * (size_t)n);
+  return BBTS_OK;
+}
+
 int bbts_hip_costmodel_grad(bbts_costmodel *cm, const float *X, const float *y, int n, float *grads, float *loss,
                             bbts_stream_t stream) {
   BBTS_REQUIRE(cm && X && y && grads && loss && n >= 1, BBTS_E_INVALID, "bad grad arguments");

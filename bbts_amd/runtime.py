@@ -1,0 +1,185 @@
+"""ctypes binding of libbbts_runtime.so (include/bbts_runtime.h): the C++ host
+runtime mirror -- udf manager, HBM storage, reservation station, command runner.
+"""
+import ctypes
+import os
+import struct
+
+import numpy as np
+
+from . import BBTSError, META_BYTES, lib
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+RT_PATH = os.path.join(_HERE, "libbbts_runtime.so")
+
+SERIAL_LIFO, BATCHED, SERIAL_LEARNED = 0, 1, 2
+
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_pi32 = ctypes.POINTER(ctypes.c_int32)
+_pu32 = ctypes.POINTER(ctypes.c_uint32)
+_pi64 = ctypes.POINTER(ctypes.c_int64)
+_SIGS = {
+    "bbts_rt_last_error": [],
+    "bbts_rt_create": [_vp, ctypes.POINTER(_vp)],
+    "bbts_rt_destroy": [_vp],
+    "bbts_rt_find_impl": [_vp, ctypes.c_char_p, ctypes.c_int, _pi32, _pi32],
+    "bbts_rt_format_id": [_vp, ctypes.c_char_p, _pi32],
+    "bbts_rt_put_dense": [_vp, _i32, ctypes.c_uint32, ctypes.c_uint32, _vp],
+    "bbts_rt_put_cu": [_vp, _i32, ctypes.c_int, _pi64, _vp],
+    "bbts_rt_has": [_vp, _i32],
+    "bbts_rt_meta": [_vp, _i32, _vp, ctypes.POINTER(ctypes.c_size_t)],
+    "bbts_rt_get_bytes": [_vp, _i32, _vp, ctypes.c_size_t],
+    "bbts_rt_num_tensors": [_vp],
+    "bbts_rt_apply": [_vp, _i32, _i32, _i32, _pi32, ctypes.c_int, _pi32, ctypes.c_int, _pu32, ctypes.c_int],
+    "bbts_rt_reduce": [_vp, _i32, _i32, _i32, _pi32, ctypes.c_int, _i32, _pu32, ctypes.c_int],
+    "bbts_rt_broadcast": [_vp, _i32, _i32, _pi32, ctypes.c_int],
+    "bbts_rt_delete": [_vp, _i32, _pi32, ctypes.c_int],
+    "bbts_rt_run": [_vp, ctypes.c_int, ctypes.c_int, _vp],
+    "bbts_rt_stats": [_vp] + [_pi64] * 4,
+    "bbts_rt_retire_order": [_vp, _pi32, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+}
+_lib = None
+
+
+def declared_symbols():
+    return sorted(_SIGS)
+
+
+def rtlib():
+    global _lib
+    if _lib is None:
+        lib()  # libbbts_hip.so first (RTLD_GLOBAL)
+        if not os.path.exists(RT_PATH):
+            raise RuntimeError("libbbts_runtime.so is not built (run: python -m bbts_amd.build)")
+        L = ctypes.CDLL(RT_PATH)
+        for name, args in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_char_p if name == "bbts_rt_last_error" else ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise BBTSError(rc, rtlib().bbts_rt_last_error().decode(errors="replace"))
+
+
+def _arr(vals, t=ctypes.c_int32):
+    vals = list(vals)
+    return (t * max(1, len(vals)))(*vals), len(vals)
+
+
+def params_words(params):
+    """command_param_t words from Python values: float -> f, bool -> b, int -> i/u."""
+    words = []
+    for p in params:
+        if isinstance(p, bool):
+            words.append(int(p))
+        elif isinstance(p, float):
+            words.append(struct.unpack("<I", struct.pack("<f", p))[0])
+        else:
+            words.append(int(p) & 0xFFFFFFFF)
+    return words
+
+
+class Runtime:
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        _check(rtlib().bbts_rt_create(ctx.h, ctypes.byref(h)))
+        self.h = h
+        self._next_cmd = 0
+
+    def close(self):
+        if self.h:
+            _check(rtlib().bbts_rt_destroy(self.h))
+            self.h = None
+
+    def impl(self, ud_name, is_gpu=True):
+        u, i = ctypes.c_int32(), ctypes.c_int32()
+        _check(rtlib().bbts_rt_find_impl(self.h, ud_name.encode(), int(is_gpu), ctypes.byref(u), ctypes.byref(i)))
+        return (u.value, i.value)
+
+    def format_id(self, name):
+        f = ctypes.c_int32()
+        _check(rtlib().bbts_rt_format_id(self.h, name.encode(), ctypes.byref(f)))
+        return f.value
+
+    def put_dense(self, tid, data):
+        a = np.ascontiguousarray(data, dtype=np.float32)
+        _check(rtlib().bbts_rt_put_dense(self.h, tid, a.shape[0], a.shape[1], a.ctypes.data))
+
+    def put_cu(self, tid, dims, data):
+        a = np.ascontiguousarray(data, dtype=np.float32)
+        d, n = _arr(dims, ctypes.c_int64)
+        _check(rtlib().bbts_rt_put_cu(self.h, tid, n, d, a.ctypes.data))
+
+    def has(self, tid):
+        return bool(rtlib().bbts_rt_has(self.h, tid))
+
+    def meta(self, tid):
+        buf = ctypes.create_string_buffer(META_BYTES)
+        nb = ctypes.c_size_t()
+        _check(rtlib().bbts_rt_meta(self.h, tid, buf, ctypes.byref(nb)))
+        return bytes(buf.raw), nb.value
+
+    def get_bytes(self, tid):
+        _, nb = self.meta(tid)
+        out = np.empty(nb, dtype=np.uint8)
+        _check(rtlib().bbts_rt_get_bytes(self.h, tid, out.ctypes.data, nb))
+        return out
+
+    def get_dense(self, tid):
+        raw = self.get_bytes(tid)
+        _, r, c = struct.unpack_from("<iII", raw.tobytes(), 0)
+        return raw[META_BYTES:META_BYTES + 4 * r * c].view(np.float32).reshape(r, c).copy()
+
+    def num_tensors(self):
+        return rtlib().bbts_rt_num_tensors(self.h)
+
+    def _id(self):
+        self._next_cmd += 1
+        return self._next_cmd
+
+    def apply(self, fun, ins, outs, params=()):
+        i, ni = _arr(ins)
+        o, no = _arr(outs)
+        w, nw = _arr(params_words(params), ctypes.c_uint32)
+        cid = self._id()
+        _check(rtlib().bbts_rt_apply(self.h, cid, fun[0], fun[1], i, ni, o, no, w, nw))
+        return cid
+
+    def reduce(self, fun, ins, out, params=()):
+        i, ni = _arr(ins)
+        w, nw = _arr(params_words(params), ctypes.c_uint32)
+        cid = self._id()
+        _check(rtlib().bbts_rt_reduce(self.h, cid, fun[0], fun[1], i, ni, out, w, nw))
+        return cid
+
+    def broadcast(self, src, dsts):
+        d, n = _arr(dsts)
+        cid = self._id()
+        _check(rtlib().bbts_rt_broadcast(self.h, cid, src, d, n))
+        return cid
+
+    def delete(self, tids):
+        t, n = _arr(tids)
+        cid = self._id()
+        _check(rtlib().bbts_rt_delete(self.h, cid, t, n))
+        return cid
+
+    def run(self, policy=SERIAL_LIFO, precision=0, costmodel=None):
+        _check(rtlib().bbts_rt_run(self.h, policy, precision, costmodel.h if costmodel is not None else None))
+
+    def stats(self):
+        v = [ctypes.c_int64() for _ in range(4)]
+        _check(rtlib().bbts_rt_stats(self.h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(["retired", "ud_calls", "batches", "batched_ops"], [x.value for x in v]))
+
+    def retire_order(self):
+        n = ctypes.c_int()
+        buf = (ctypes.c_int32 * 100000)()
+        _check(rtlib().bbts_rt_retire_order(self.h, buf, 100000, ctypes.byref(n)))
+        return list(buf[:n.value])

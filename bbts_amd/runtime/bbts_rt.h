@@ -1,0 +1,311 @@
+// C++ host runtime mirror of the reference's per-node interfaces, on HBM blocks.
+//
+// Kept from the reference (names, argument meaning, registration order):
+//   tensor_meta_t / dense_tensor_meta_t / cu_meta_t ..... src/tensor/tensor.h:20-67,
+//                                                         builtin_formats.h:7-45, libraries/cutensor/cu.h:60-99
+//   tensor_factory_t .................................... src/tensor/tensor_factory.cc:5-68
+//   command_param_t, ud_impl_t, ud_func_t ............... src/commands/command_utils.h:8-16,
+//                                                         src/ud_functions/ud_function.h:17-265
+//   udf_manager_t ....................................... src/ud_functions/udf_manager.cc:40-171
+//   storage (tid map, anonymous tids, assign_tid) ....... src/storage/memory_storage.{h,cc}
+//   reservation_station_t ............................... src/commands/reservation_station.cc (single node)
+//   command_runner_t (APPLY / local REDUCE / MOVE / DELETE) src/commands/command_runner.cc:86-210
+// Changed for MI355X:
+//   * tensor_t* is the device address of a block in HBM; the host reads headers
+//     from the storage's host mirror (libbbts_hip keeps one per block).
+//   * tensor_params_t carries {stream, ctx, precision} in place of the CUDA
+//     stream / cuBLAS / cuTENSOR handles (ud_function.h:158-164).
+//   * There is no GPU scheduler thread: ud calls are enqueued on one HIP stream
+//     in retirement order, so stream order carries the data dependencies, and a
+//     command retires when its kernels are enqueued.
+//   * Complexity hints are computed in 64 bits (the reference's uint32 products
+//     wrap for 2048^3, dense_matrix_mult.cc:32).
+#pragma once
+
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "bbts_hip.h"
+#include "bbts_runtime.h"
+
+namespace bbts {
+
+using tid_t = int32_t;
+const tid_t TID_NONE = -1;
+using tfid_t = int32_t;
+
+// ---- tensor layout ---------------------------------------------------------
+struct tensor_meta_t {
+  static const size_t MAX_META_SIZE = 256;
+  template <class T>
+  T &as() const { return *((T *)this); }
+  tfid_t fmt_id;
+  char _blob[MAX_META_SIZE];
+};
+static_assert(sizeof(tensor_meta_t) == 260, "tensor_meta_t is 260 bytes");
+
+// A block in HBM: the device address of its header (never dereferenced on the host).
+using tensor_t = bbts_tensor_t;
+
+struct dense_tensor_meta_t : public tensor_meta_t {
+  struct m_t {
+    uint32_t num_rows;
+    uint32_t num_cols;
+  };
+  m_t &m() const { return *((m_t *)_blob); }
+};
+
+struct cu_shape_t {
+  uint32_t rank;
+  int64_t dims[4];
+};
+struct cu_meta_t : public tensor_meta_t {
+  cu_shape_t &m() const { return *((cu_shape_t *)_blob); }
+};
+
+// ---- tensor factory ----------------------------------------------------------
+struct tensor_creation_fs_t {
+  std::function<size_t(const tensor_meta_t &)> get_size;
+};
+
+class tensor_factory_t {
+ public:
+  tensor_factory_t();
+  tfid_t register_fmt(const std::string &name, const tensor_creation_fs_t &fs);
+  tfid_t get_tensor_ftm(const std::string &name) const;
+  size_t get_tensor_size(const tensor_meta_t &meta) const;
+
+ private:
+  std::vector<tensor_creation_fs_t> _fmts;
+  std::unordered_map<std::string, tfid_t> _ids;
+};
+
+// ---- ud functions ------------------------------------------------------------
+union command_param_t {
+  float f;
+  int32_t i;
+  uint32_t u;
+  bool b;
+};
+static_assert(sizeof(command_param_t) == 4, "command_param_t is one 32-bit word");
+using command_param_list_t = std::vector<command_param_t>;
+
+using ud_id_t = int32_t;
+using ud_impl_local_t = int32_t;
+struct ud_impl_id_t {
+  ud_id_t ud_id;
+  ud_impl_local_t impl_id;
+};
+
+struct ud_impl_t {
+  template <class T>
+  struct ud_impl_args_t {
+    ud_impl_args_t() = default;
+    explicit ud_impl_args_t(std::vector<T *> a) : arguments(std::move(a)) {}
+    explicit ud_impl_args_t(size_t n) : arguments(n) {}
+    size_t num_args() const { return arguments.size(); }
+    template <size_t n>
+    T &get() { return *arguments[n]; }
+    template <size_t n>
+    const T &get() const { return *arguments[n]; }
+    T *ptr(size_t idx) const { return arguments[idx]; }
+    const T &get_by_idx(size_t idx) const { return *arguments[idx]; }
+    T &get_by_idx(size_t idx) { return *arguments[idx]; }
+    void set(size_t n, T *t) { arguments[n] = t; }
+
+   private:
+    std::vector<T *> arguments;
+  };
+  // inputs / outputs hold device block addresses; metas are host copies
+  using tensor_args_t = ud_impl_args_t<tensor_t>;
+  using meta_args_t = ud_impl_args_t<tensor_meta_t>;
+
+  struct tensor_params_t {
+    template <size_t n>
+    float get_float() const { return _params[n].f; }
+    template <size_t n>
+    int32_t get_int() const { return _params[n].i; }
+    template <size_t n>
+    uint32_t get_uint() const { return _params[n].u; }
+    template <size_t n>
+    uint32_t get_bool() const { return _params[n].b; }
+    template <size_t n>
+    float get_float_or_default(float v) const { return n < _params.size() ? _params[n].f : v; }
+    template <size_t n>
+    int32_t get_int_or_default(int32_t v) const { return n < _params.size() ? _params[n].i : v; }
+    template <size_t n>
+    uint32_t get_uint_or_default(uint32_t v) const { return n < _params.size() ? _params[n].u : v; }
+    template <size_t n>
+    uint32_t get_bool_or_default(bool v) const { return n < _params.size() ? _params[n].b : v; }
+    command_param_t get_raw(size_t n) const { return _params[n]; }
+    size_t num_parameters() const { return _params.size(); }
+
+    command_param_list_t _params;
+    bbts_stream_t stream = nullptr;   // in place of cudaStream_t
+    bbts_hip_ctx *ctx = nullptr;      // in place of the cuBLAS / cuTENSOR handles
+    int precision = BBTS_PREC_FP32;   // matrix_mult / contraction precision mode
+  };
+
+  using ud_impl_callable =
+      std::function<void(const tensor_params_t &params, const tensor_args_t &_in, tensor_args_t &_out)>;
+
+  ud_impl_id_t impl_id{-1, -1};
+  std::string impl_name;
+  std::string ud_name;
+  std::vector<std::string> inputTypes;
+  std::vector<int32_t> inputInplace;
+  std::vector<std::string> outputTypes;
+  bool is_gpu = true;
+
+  virtual ~ud_impl_t() = default;
+  void call_ud(const tensor_params_t &p, const tensor_args_t &in, tensor_args_t &out) const { fn(p, in, out); }
+  ud_impl_callable fn;
+  ud_impl_callable gpu_fn;
+  virtual size_t get_complexity_hint(const tensor_params_t &params, const meta_args_t &_in) = 0;
+  virtual void get_out_meta(const tensor_params_t &params, const meta_args_t &_in, meta_args_t &_out) const = 0;
+};
+using ud_impl_ptr_t = std::unique_ptr<ud_impl_t>;
+
+struct ud_func_t {
+  ud_id_t id = -1;
+  std::string ud_name;
+  bool is_ass = false;
+  bool is_com = false;
+  size_t num_in = 0;
+  size_t num_out = 0;
+  std::vector<ud_impl_ptr_t> impls;
+  ud_impl_id_t add_impl(ud_impl_ptr_t impl) {
+    auto local = static_cast<ud_impl_local_t>(impls.size());
+    impl->impl_id = ud_impl_id_t{id, local};
+    impls.emplace_back(std::move(impl));
+    return impls.back()->impl_id;
+  }
+};
+using ud_func_ptr_t = std::unique_ptr<ud_func_t>;
+
+class udf_manager_t {
+ public:
+  explicit udf_manager_t(std::shared_ptr<tensor_factory_t> factory);
+  ud_id_t register_udf(ud_func_ptr_t fun);
+  ud_impl_id_t register_udf_impl(ud_impl_ptr_t impl);
+  ud_impl_t *get_fn_impl(ud_impl_id_t id) const;
+  ud_func_t *get_udf(const std::string &name) const;
+  // first impl of `ud_name` whose types match and whose is_gpu flag equals require_gpu
+  ud_impl_t *find_match(const std::string &ud_name, const std::vector<std::string> &in,
+                        const std::vector<std::string> &out, bool require_gpu) const;
+
+ private:
+  std::shared_ptr<tensor_factory_t> _factory;
+  std::vector<ud_func_ptr_t> _udfs;
+  std::unordered_map<std::string, ud_id_t> _udfs_name_to_id;
+};
+
+// The cutensor library plugin (libraries/cutensor.cc:16-32): registers the
+// "cutensor" format and the "contraction" ud backed by bbts_hip_contraction.
+void register_cutensor_library(tensor_factory_t &factory, udf_manager_t &udm);
+
+// ---- storage -----------------------------------------------------------------
+class hbm_storage_t {
+ public:
+  explicit hbm_storage_t(bbts_hip_ctx *ctx) : _ctx(ctx) {}
+  ~hbm_storage_t();
+  // create a tensor (tid == TID_NONE: next anonymous tid, -2, -3, ...); returns its tid
+  tid_t create(tid_t tid, const tensor_meta_t &meta, size_t bytes, tensor_t **blk);
+  tensor_t *get(tid_t tid) const;
+  bool has(tid_t tid) const;
+  tensor_meta_t meta(tid_t tid) const;
+  size_t bytes(tid_t tid) const;
+  void remove_by_tid(tid_t tid);
+  void assign_tid(tid_t from, tid_t to);
+  size_t size() const;
+  bbts_hip_ctx *ctx() const { return _ctx; }
+
+ private:
+  struct entry {
+    tensor_t *blk;
+    size_t bytes;
+  };
+  bbts_hip_ctx *_ctx;
+  mutable std::mutex _m;
+  std::unordered_map<tid_t, entry> _tensors;
+  tid_t _next_anon = -2;
+};
+
+// ---- commands ----------------------------------------------------------------
+struct command_t {
+  enum op_type_t { APPLY = 0, REDUCE = 1, MOVE = 2, DELETE = 3 };
+  int32_t id = -1;
+  op_type_t type = APPLY;
+  ud_impl_id_t fun_id{-1, -1};
+  std::vector<tid_t> inputs;
+  std::vector<tid_t> outputs;
+  command_param_list_t params;
+};
+
+// Dependency tracking on one node: a command is ready once its inputs exist;
+// a DELETE waits until every earlier reader of its tensors has retired.
+class reservation_station_t {
+ public:
+  void queue_command(const command_t &cmd, const std::function<bool(tid_t)> &exists);
+  bool empty() const { return _pending.empty(); }
+  std::vector<int32_t> &ready() { return _ready; }
+  const command_t &command(int32_t id) const { return _pending.at(id).cmd; }
+  void retire(int32_t id);
+
+ private:
+  struct state {
+    command_t cmd;
+    int missing = 0;
+  };
+  void make_ready_if(int32_t id);
+  std::map<int32_t, state> _pending;
+  std::vector<int32_t> _ready;
+  std::unordered_map<tid_t, std::vector<int32_t>> _waiting;  // tid -> commands waiting for it
+  std::unordered_map<tid_t, int> _readers;                   // queued commands reading tid
+  std::unordered_map<tid_t, std::vector<int32_t>> _deletes;  // tid -> deletes waiting for readers
+  std::set<tid_t> _will_exist;
+  int32_t _last_id = -1;
+};
+
+class command_runner_t {
+ public:
+  command_runner_t(hbm_storage_t &storage, udf_manager_t &udm, tensor_factory_t &factory, bbts_hip_ctx *ctx);
+  void run(reservation_station_t &rs, int policy, int precision, bbts_costmodel *cm);
+  int64_t retired = 0, ud_calls = 0, batches = 0, batched_ops = 0;
+  std::vector<int32_t> retire_order;
+
+ private:
+  void run_apply(const command_t &cmd, int precision);
+  void run_reduce(const command_t &cmd);
+  void run_move(const command_t &cmd);
+  void run_delete(const command_t &cmd);
+  void run_matmul_batch(const std::vector<const command_t *> &cmds, int precision, bbts_costmodel *cm);
+  void create_outputs(const command_t &cmd, ud_impl_t *ud, const ud_impl_t::tensor_params_t &params,
+                      std::vector<tensor_t *> &outs);
+  float predicted_cost(const command_t &cmd, bbts_costmodel *cm);
+  hbm_storage_t &_storage;
+  udf_manager_t &_udm;
+  tensor_factory_t &_factory;
+  bbts_hip_ctx *_ctx;
+  bbts_stream_t _stream = nullptr;
+};
+
+// Throwing wrapper of a libbbts_hip status (the reference surfaces ud errors as exceptions).
+struct hip_error : std::runtime_error {
+  int code;
+  hip_error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+inline void check(int rc) {
+  if (rc != BBTS_OK) throw hip_error(rc, bbts_hip_last_error());
+}
+
+}  // namespace bbts

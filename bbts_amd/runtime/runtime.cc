@@ -1,0 +1,902 @@
+// Host runtime mirror: factory, ud impls over libbbts_hip, udf manager,
+// HBM storage, reservation station, command runner, and the bbts_rt_* C ABI.
+#include <algorithm>
+#include <cmath>
+
+#include "bbts_rt.h"
+
+namespace bbts {
+
+// ============================================================================
+// tensor_factory_t (src/tensor/tensor_factory.cc:5-68): "dense" is format 0.
+// ============================================================================
+tensor_factory_t::tensor_factory_t() {
+  register_fmt("dense", tensor_creation_fs_t{[](const tensor_meta_t &m) {
+                 const auto &d = m.as<dense_tensor_meta_t>().m();
+                 return sizeof(tensor_meta_t) + (size_t)d.num_rows * (size_t)d.num_cols * sizeof(float);
+               }});
+}
+
+tfid_t tensor_factory_t::register_fmt(const std::string &name, const tensor_creation_fs_t &fs) {
+  auto it = _ids.find(name);
+  if (it != _ids.end()) return it->second;
+  tfid_t id = (tfid_t)_fmts.size();
+  _fmts.push_back(fs);
+  _ids[name] = id;
+  return id;
+}
+
+tfid_t tensor_factory_t::get_tensor_ftm(const std::string &name) const {
+  auto it = _ids.find(name);
+  return it == _ids.end() ? -1 : it->second;
+}
+
+size_t tensor_factory_t::get_tensor_size(const tensor_meta_t &meta) const {
+  if (meta.fmt_id < 0 || (size_t)meta.fmt_id >= _fmts.size())
+    throw std::runtime_error("unknown tensor format id " + std::to_string(meta.fmt_id));
+  return _fmts[(size_t)meta.fmt_id].get_size(meta);
+}
+
+// ============================================================================
+// ud impls on libbbts_hip
+// ============================================================================
+namespace {
+
+using tensor_params_t = ud_impl_t::tensor_params_t;
+using tensor_args_t = ud_impl_t::tensor_args_t;
+using meta_args_t = ud_impl_t::meta_args_t;
+
+// matrix_add (dense_matrix_add.cc:4-77 / dense_matrix_gpu_add.cu:4-85)
+struct hip_dense_add_t : public ud_impl_t {
+  hip_dense_add_t(const char *name, bool gpu_slot) {
+    impl_name = name;
+    ud_name = "matrix_add";
+    inputTypes = {"dense", "dense"};
+    outputTypes = {"dense"};
+    inputInplace = {0, 1};
+    is_gpu = gpu_slot;
+    fn = [](const tensor_params_t &p, const tensor_args_t &in, tensor_args_t &out) {
+      check(bbts_hip_dense_add(p.ctx, in.ptr(0), in.ptr(1), out.ptr(0), p.stream));
+    };
+  }
+  size_t get_complexity_hint(const tensor_params_t &, const meta_args_t &in) override {
+    const auto &a = in.get<0>().as<dense_tensor_meta_t>().m();
+    return (size_t)a.num_rows * a.num_cols;
+  }
+  void get_out_meta(const tensor_params_t &, const meta_args_t &in, meta_args_t &out) const override {
+    const auto &a = in.get<0>().as<dense_tensor_meta_t>().m();
+    out.get<0>().as<dense_tensor_meta_t>().m() = {a.num_rows, a.num_cols};
+  }
+};
+
+// matrix_mult (dense_matrix_mult.cc:6-86 semantics; replaces dense_matrix_gpu_mult.cu)
+struct hip_dense_mult_t : public ud_impl_t {
+  hip_dense_mult_t(const char *name, bool gpu_slot) {
+    impl_name = name;
+    ud_name = "matrix_mult";
+    inputTypes = {"dense", "dense"};
+    outputTypes = {"dense"};
+    inputInplace = {};
+    is_gpu = gpu_slot;
+    fn = [](const tensor_params_t &p, const tensor_args_t &in, tensor_args_t &out) {
+      int ta = (int)p.get_bool_or_default<0>(false), tb = (int)p.get_bool_or_default<1>(false);
+      check(bbts_hip_dense_matmul(p.ctx, in.ptr(0), in.ptr(1), out.ptr(0), ta, tb, p.precision, p.stream));
+    };
+  }
+  size_t get_complexity_hint(const tensor_params_t &, const meta_args_t &in) override {
+    const auto &a = in.get<0>().as<dense_tensor_meta_t>().m();
+    const auto &b = in.get<1>().as<dense_tensor_meta_t>().m();
+    return (size_t)a.num_rows * a.num_cols * b.num_cols;
+  }
+  void get_out_meta(const tensor_params_t &, const meta_args_t &in, meta_args_t &out) const override {
+    const auto &a = in.get<0>().as<dense_tensor_meta_t>().m();
+    const auto &b = in.get<1>().as<dense_tensor_meta_t>().m();
+    out.get<0>().as<dense_tensor_meta_t>().m() = {a.num_rows, b.num_cols};
+  }
+};
+
+// uniform (dense_uniform.cc:27-79): params {u rows, u cols, f left, f right}; an optional fifth
+// word is the seed (the reference seeds with time(), which is not reproducible).
+struct hip_dense_uniform_t : public ud_impl_t {
+  hip_dense_uniform_t() {
+    impl_name = "dense_uniform";
+    ud_name = "uniform";
+    inputTypes = {};
+    outputTypes = {"dense"};
+    inputInplace = {};
+    is_gpu = false;  // the reference's only impl is the CPU one (impl 0); here it fills HBM
+    fn = [](const tensor_params_t &p, const tensor_args_t &, tensor_args_t &out) {
+      uint32_t rows = p.get_uint<0>(), cols = p.get_uint<1>();
+      float left = p.get_float_or_default<2>(0.0f), right = p.get_float_or_default<3>(1.0f);
+      uint64_t seed = p.get_uint_or_default<4>(0u);
+      check(bbts_hip_dense_uniform(p.ctx, out.ptr(0), rows, cols, left, right, seed, p.stream));
+    };
+  }
+  size_t get_complexity_hint(const tensor_params_t &p, const meta_args_t &) override {
+    return (size_t)p.get_uint<0>() * p.get_uint<1>();
+  }
+  void get_out_meta(const tensor_params_t &p, const meta_args_t &, meta_args_t &out) const override {
+    out.get<0>().as<dense_tensor_meta_t>().m() = {p.get_uint<0>(), p.get_uint<1>()};
+  }
+};
+
+// matrix_relu (applications/fnn/ffnn_relu.cc:46-71 semantics): an extra ud of this build
+struct hip_dense_relu_t : public ud_impl_t {
+  hip_dense_relu_t() {
+    impl_name = "dense_matrix_hip_relu";
+    ud_name = "matrix_relu";
+    inputTypes = {"dense"};
+    outputTypes = {"dense"};
+    inputInplace = {0};
+    is_gpu = true;
+    fn = [](const tensor_params_t &p, const tensor_args_t &in, tensor_args_t &out) {
+      check(bbts_hip_dense_relu(p.ctx, in.ptr(0), out.ptr(0), p.stream));
+    };
+  }
+  size_t get_complexity_hint(const tensor_params_t &, const meta_args_t &in) override {
+    const auto &a = in.get<0>().as<dense_tensor_meta_t>().m();
+    return (size_t)a.num_rows * a.num_cols;
+  }
+  void get_out_meta(const tensor_params_t &, const meta_args_t &in, meta_args_t &out) const override {
+    const auto &a = in.get<0>().as<dense_tensor_meta_t>().m();
+    out.get<0>().as<dense_tensor_meta_t>().m() = {a.num_rows, a.num_cols};
+  }
+};
+
+// contraction (libraries/cutensor/contraction.h:633-667): out meta from parse + set_out_meta
+struct hip_contraction_t : public ud_impl_t {
+  hip_contraction_t() {
+    impl_name = "contraction";
+    ud_name = "contraction";
+    inputTypes = {"cutensor", "cutensor"};
+    outputTypes = {"cutensor"};
+    inputInplace = {};
+    is_gpu = true;
+    fn = [](const tensor_params_t &p, const tensor_args_t &in, tensor_args_t &out) {
+      std::vector<int32_t> words(p.num_parameters());
+      for (size_t i = 0; i < words.size(); ++i) words[i] = p._params[i].i;
+      check(bbts_hip_contraction(p.ctx, in.ptr(0), in.ptr(1), out.ptr(0), words.data(), (int)words.size(),
+                                 p.precision, p.stream));
+    };
+  }
+  // incident dims: max mode id + 1 over both inputs; lhs then rhs extents (parse, :25-56)
+  static std::vector<int64_t> incident(const tensor_params_t &p, const meta_args_t &in) {
+    const auto &l = in.get<0>().as<cu_meta_t>().m();
+    const auto &r = in.get<1>().as<cu_meta_t>().m();
+    int rl = p._params.at(0).i, rr = p._params.at(1).i;
+    int m = 0;
+    for (int x = 0; x < rl + rr; ++x) m = std::max(m, 1 + p._params.at(3 + x).i);
+    std::vector<int64_t> dims((size_t)m, 0);
+    for (int x = 0; x < rl; ++x) dims[(size_t)p._params[3 + x].i] = l.dims[x];
+    for (int x = 0; x < rr; ++x) dims[(size_t)p._params[3 + rl + x].i] = r.dims[x];
+    return dims;
+  }
+  size_t get_complexity_hint(const tensor_params_t &p, const meta_args_t &in) override {
+    size_t h = 1;
+    for (auto d : incident(p, in)) h *= (size_t)d;
+    return h;
+  }
+  void get_out_meta(const tensor_params_t &p, const meta_args_t &in, meta_args_t &out) const override {
+    auto dims = incident(p, in);
+    int rl = p._params.at(0).i, rr = p._params.at(1).i, ro = p._params.at(2).i;
+    auto &o = out.get<0>().as<cu_meta_t>().m();
+    o.rank = (uint32_t)ro;
+    for (int x = 0; x < ro; ++x) o.dims[x] = dims.at((size_t)p._params.at(3 + rl + rr + x).i);
+  }
+};
+
+}  // namespace
+
+// ============================================================================
+// udf_manager_t (udf_manager.cc:40-74): registration order fixes the impl ids
+// that compiled commands carry (command.h:466):
+//   matrix_add  = ud 0: impl 0 "dense_matrix_add"  (host slot), impl 1 GPU
+//   matrix_mult = ud 1: impl 0 "dense_matrix_mult" (host slot), impl 1 GPU
+//   uniform     = ud 2: impl 0
+//   matrix_relu = ud 3 (this build's addition)
+// Blocks live in HBM, so the host-slot impls run the same HIP kernels; a
+// command compiled for either slot dispatches unchanged.
+// ============================================================================
+udf_manager_t::udf_manager_t(std::shared_ptr<tensor_factory_t> factory) : _factory(std::move(factory)) {
+  auto mk = [](const char *name, bool ass, bool com, size_t nin, size_t nout) {
+    auto f = std::make_unique<ud_func_t>();
+    f->ud_name = name;
+    f->is_ass = ass;
+    f->is_com = com;
+    f->num_in = nin;
+    f->num_out = nout;
+    return f;
+  };
+  register_udf(mk("matrix_add", true, true, 2, 1));  // builtin_functions.cc:4-15
+  register_udf_impl(std::make_unique<hip_dense_add_t>("dense_matrix_add", false));
+  register_udf_impl(std::make_unique<hip_dense_add_t>("dense_matrix_hip_add", true));
+  register_udf(mk("matrix_mult", true, false, 2, 1));  // builtin_functions.cc:17-28
+  register_udf_impl(std::make_unique<hip_dense_mult_t>("dense_matrix_mult", false));
+  register_udf_impl(std::make_unique<hip_dense_mult_t>("dense_matrix_hip_mult", true));
+  register_udf(mk("uniform", false, false, 0, 1));  // builtin_functions.cc:30-43
+  register_udf_impl(std::make_unique<hip_dense_uniform_t>());
+  register_udf(mk("matrix_relu", false, false, 1, 1));
+  register_udf_impl(std::make_unique<hip_dense_relu_t>());
+}
+
+ud_id_t udf_manager_t::register_udf(ud_func_ptr_t fun) {
+  if (_udfs_name_to_id.count(fun->ud_name)) return -1;
+  ud_id_t id = (ud_id_t)_udfs.size();
+  fun->id = id;
+  _udfs_name_to_id[fun->ud_name] = id;
+  _udfs.emplace_back(std::move(fun));
+  return id;
+}
+
+ud_impl_id_t udf_manager_t::register_udf_impl(ud_impl_ptr_t impl) {
+  auto it = _udfs_name_to_id.find(impl->ud_name);
+  if (it == _udfs_name_to_id.end()) return {-1, -1};
+  auto &f = _udfs[(size_t)it->second];
+  // udf_manager.cc:120-123 rejects only when both arities mismatch; kept
+  if (impl->inputTypes.size() != f->num_in && impl->outputTypes.size() != f->num_out) return {-1, -1};
+  impl->gpu_fn = impl->fn;
+  return f->add_impl(std::move(impl));
+}
+
+ud_impl_t *udf_manager_t::get_fn_impl(ud_impl_id_t id) const {
+  if (id.ud_id < 0 || (size_t)id.ud_id >= _udfs.size()) return nullptr;
+  auto &f = _udfs[(size_t)id.ud_id];
+  if (id.impl_id < 0 || (size_t)id.impl_id >= f->impls.size()) return nullptr;
+  return f->impls[(size_t)id.impl_id].get();
+}
+
+ud_func_t *udf_manager_t::get_udf(const std::string &name) const {
+  auto it = _udfs_name_to_id.find(name);
+  return it == _udfs_name_to_id.end() ? nullptr : _udfs[(size_t)it->second].get();
+}
+
+ud_impl_t *udf_manager_t::find_match(const std::string &ud_name, const std::vector<std::string> &in,
+                                     const std::vector<std::string> &out, bool require_gpu) const {
+  ud_func_t *f = get_udf(ud_name);
+  if (!f) return nullptr;
+  for (auto &impl : f->impls) {
+    if (impl->is_gpu != require_gpu) continue;
+    bool ok = true;
+    for (size_t i = 0; i < in.size() && i < impl->inputTypes.size(); ++i) ok = ok && in[i] == impl->inputTypes[i];
+    for (size_t i = 0; i < out.size() && i < impl->outputTypes.size(); ++i)
+      ok = ok && out[i] == impl->outputTypes[i];
+    if (ok) return impl.get();
+  }
+  return nullptr;
+}
+
+void register_cutensor_library(tensor_factory_t &factory, udf_manager_t &udm) {
+  factory.register_fmt("cutensor", tensor_creation_fs_t{[](const tensor_meta_t &m) {
+                         const auto &s = m.as<cu_meta_t>().m();
+                         size_t n = 1;
+                         for (uint32_t r = 0; r < s.rank && r < 4; ++r) n *= (size_t)s.dims[r];
+                         return sizeof(tensor_meta_t) + 4 * n;
+                       }});
+  auto f = std::make_unique<ud_func_t>();
+  f->ud_name = "contraction";
+  f->num_in = 2;
+  f->num_out = 1;
+  udm.register_udf(std::move(f));
+  udm.register_udf_impl(std::make_unique<hip_contraction_t>());
+}
+
+// ============================================================================
+// hbm_storage_t (memory_storage.cc:70-103, 141-157, 203-229)
+// ============================================================================
+hbm_storage_t::~hbm_storage_t() {
+  for (auto &kv : _tensors) bbts_hip_block_free(_ctx, kv.second.blk);
+}
+
+tid_t hbm_storage_t::create(tid_t tid, const tensor_meta_t &meta, size_t bytes, tensor_t **blk) {
+  tensor_t *b = nullptr;
+  check(bbts_hip_block_alloc(_ctx, &meta, bytes, &b));
+  std::lock_guard<std::mutex> g(_m);
+  if (tid == TID_NONE) tid = _next_anon--;
+  if (_tensors.count(tid)) {
+    bbts_hip_block_free(_ctx, b);
+    throw std::runtime_error("tensor " + std::to_string(tid) + " already exists");
+  }
+  _tensors[tid] = entry{b, bytes};
+  if (blk) *blk = b;
+  return tid;
+}
+
+tensor_t *hbm_storage_t::get(tid_t tid) const {
+  std::lock_guard<std::mutex> g(_m);
+  auto it = _tensors.find(tid);
+  if (it == _tensors.end()) throw std::runtime_error("tensor " + std::to_string(tid) + " does not exist");
+  return it->second.blk;
+}
+
+bool hbm_storage_t::has(tid_t tid) const {
+  std::lock_guard<std::mutex> g(_m);
+  return _tensors.count(tid) != 0;
+}
+
+tensor_meta_t hbm_storage_t::meta(tid_t tid) const {
+  tensor_meta_t m;
+  check(bbts_hip_block_meta(_ctx, get(tid), &m, nullptr));
+  return m;
+}
+
+size_t hbm_storage_t::bytes(tid_t tid) const {
+  std::lock_guard<std::mutex> g(_m);
+  return _tensors.at(tid).bytes;
+}
+
+void hbm_storage_t::remove_by_tid(tid_t tid) {
+  tensor_t *b;
+  {
+    std::lock_guard<std::mutex> g(_m);
+    auto it = _tensors.find(tid);
+    if (it == _tensors.end()) return;
+    b = it->second.blk;
+    _tensors.erase(it);
+  }
+  check(bbts_hip_block_free(_ctx, b));
+}
+
+void hbm_storage_t::assign_tid(tid_t from, tid_t to) {
+  std::lock_guard<std::mutex> g(_m);
+  auto it = _tensors.find(from);
+  if (it == _tensors.end() || _tensors.count(to)) throw std::runtime_error("assign_tid: bad tids");
+  _tensors[to] = it->second;
+  _tensors.erase(it);
+}
+
+size_t hbm_storage_t::size() const {
+  std::lock_guard<std::mutex> g(_m);
+  return _tensors.size();
+}
+
+// ============================================================================
+// reservation_station_t (reservation_station.cc: _queue_local :386-508,
+// retire_command :510-633, get_next_command :201-225), one node
+// ============================================================================
+void reservation_station_t::queue_command(const command_t &cmd, const std::function<bool(tid_t)> &exists) {
+  if (cmd.id <= _last_id) throw std::runtime_error("command ids must increase");
+  if (cmd.type != command_t::DELETE)
+    for (tid_t t : cmd.inputs)
+      if (!exists(t) && !_will_exist.count(t))
+        throw std::runtime_error("input tensor " + std::to_string(t) + " is never created");
+  _last_id = cmd.id;
+  state st;
+  st.cmd = cmd;
+  if (cmd.type == command_t::DELETE) {
+    for (tid_t t : cmd.inputs) {
+      if (_readers[t] > 0 || (_will_exist.count(t) && !exists(t))) {
+        _deletes[t].push_back(cmd.id);
+        st.missing++;
+      }
+    }
+  } else {
+    for (tid_t t : cmd.inputs) {
+      _readers[t]++;
+      if (!exists(t)) {
+        _waiting[t].push_back(cmd.id);
+        st.missing++;
+      }
+    }
+    for (tid_t t : cmd.outputs) _will_exist.insert(t);
+  }
+  _pending[cmd.id] = st;
+  make_ready_if(cmd.id);
+}
+
+void reservation_station_t::make_ready_if(int32_t id) {
+  auto &st = _pending.at(id);
+  if (st.missing == 0) _ready.push_back(id);
+}
+
+void reservation_station_t::retire(int32_t id) {
+  auto it = _pending.find(id);
+  if (it == _pending.end()) return;
+  command_t cmd = it->second.cmd;
+  _pending.erase(it);
+  if (cmd.type == command_t::DELETE) return;
+  for (tid_t t : cmd.outputs) {
+    auto w = _waiting.find(t);
+    if (w != _waiting.end()) {
+      for (int32_t c : w->second) {
+        auto &st = _pending.at(c);
+        if (--st.missing == 0) _ready.push_back(c);
+      }
+      _waiting.erase(w);
+    }
+  }
+  for (tid_t t : cmd.inputs) {
+    if (--_readers[t] == 0) {
+      auto d = _deletes.find(t);
+      if (d != _deletes.end()) {
+        for (int32_t c : d->second) {
+          auto &st = _pending.at(c);
+          if (--st.missing == 0) _ready.push_back(c);
+        }
+        _deletes.erase(d);
+      }
+    }
+  }
+}
+
+// ============================================================================
+// command_runner_t
+// ============================================================================
+command_runner_t::command_runner_t(hbm_storage_t &storage, udf_manager_t &udm, tensor_factory_t &factory,
+                                   bbts_hip_ctx *ctx)
+    : _storage(storage), _udm(udm), _factory(factory), _ctx(ctx) {}
+
+void command_runner_t::create_outputs(const command_t &cmd, ud_impl_t *ud, const ud_impl_t::tensor_params_t &params,
+                                      std::vector<tensor_t *> &outs) {
+  // command_runner.cc:105-161: input metas -> get_out_meta -> fmt_id from outputTypes -> size -> create
+  std::vector<tensor_meta_t> in_meta(cmd.inputs.size()), out_meta(cmd.outputs.size());
+  std::vector<tensor_meta_t *> ip, op;
+  for (size_t i = 0; i < cmd.inputs.size(); ++i) {
+    in_meta[i] = _storage.meta(cmd.inputs[i]);
+    ip.push_back(&in_meta[i]);
+  }
+  for (auto &m : out_meta) {
+    std::memset(&m, 0, sizeof(m));
+    op.push_back(&m);
+  }
+  ud_impl_t::meta_args_t ia(ip), oa(op);
+  ud->get_out_meta(params, ia, oa);
+  outs.clear();
+  for (size_t i = 0; i < cmd.outputs.size(); ++i) {
+    out_meta[i].fmt_id = _factory.get_tensor_ftm(ud->outputTypes.at(i));
+    size_t bytes = _factory.get_tensor_size(out_meta[i]);
+    tensor_t *b = nullptr;
+    _storage.create(cmd.outputs[i], out_meta[i], bytes, &b);
+    outs.push_back(b);
+  }
+}
+
+void command_runner_t::run_apply(const command_t &cmd, int precision) {
+  ud_impl_t *ud = _udm.get_fn_impl(cmd.fun_id);
+  if (!ud) throw std::runtime_error("unknown ud impl in command " + std::to_string(cmd.id));
+  ud_impl_t::tensor_params_t params;
+  params._params = cmd.params;
+  params.stream = _stream;
+  params.ctx = _ctx;
+  params.precision = precision;
+  std::vector<tensor_t *> outs;
+  create_outputs(cmd, ud, params, outs);
+  std::vector<tensor_t *> ins;
+  for (tid_t t : cmd.inputs) ins.push_back(_storage.get(t));
+  ud_impl_t::tensor_args_t ia(ins), oa(outs);
+  ud->call_ud(params, ia, oa);
+  ud_calls++;
+}
+
+// local REDUCE (command_runner.cc:179-210 -> local_reduce_op.cc:23-91): with the
+// matrix_add ud the k-1 pairwise adds collapse into one fused left-fold kernel
+// (bit-identical); other uds fold pairwise through anonymous temporaries.
+void command_runner_t::run_reduce(const command_t &cmd) {
+  ud_impl_t *ud = _udm.get_fn_impl(cmd.fun_id);
+  if (!ud) throw std::runtime_error("unknown ud impl in reduce " + std::to_string(cmd.id));
+  const tid_t out_tid = cmd.outputs.at(0);
+  if (cmd.inputs.size() == 1) {  // local_reduce_op.cc:27,90: a single input is renamed
+    _storage.assign_tid(cmd.inputs[0], out_tid);
+    return;
+  }
+  ud_impl_t::tensor_params_t params;
+  params._params = cmd.params;
+  params.stream = _stream;
+  params.ctx = _ctx;
+  if (ud->ud_name == "matrix_add") {
+    command_t two = cmd;
+    two.inputs = {cmd.inputs[0], cmd.inputs[1]};
+    std::vector<tensor_t *> outs;
+    create_outputs(two, ud, params, outs);
+    std::vector<tensor_t *> ins;
+    for (tid_t t : cmd.inputs) ins.push_back(_storage.get(t));
+    check(bbts_hip_dense_reduce(_ctx, ins.data(), (int)ins.size(), outs[0], _stream));
+    ud_calls++;
+    return;
+  }
+  tid_t lhs = cmd.inputs[0];
+  for (size_t idx = 1; idx < cmd.inputs.size(); ++idx) {
+    command_t step;
+    step.fun_id = cmd.fun_id;
+    step.params = cmd.params;
+    step.inputs = {lhs, cmd.inputs[idx]};
+    step.outputs = {idx + 1 == cmd.inputs.size() ? out_tid : TID_NONE};
+    std::vector<tensor_t *> outs;
+    // anonymous temporaries for the intermediate steps
+    std::vector<tensor_meta_t> im(2), om(1);
+    std::vector<tensor_meta_t *> ip{&im[0], &im[1]}, op{&om[0]};
+    im[0] = _storage.meta(step.inputs[0]);
+    im[1] = _storage.meta(step.inputs[1]);
+    std::memset(&om[0], 0, sizeof(om[0]));
+    ud_impl_t::meta_args_t ia(ip), oa(op);
+    ud->get_out_meta(params, ia, oa);
+    om[0].fmt_id = _factory.get_tensor_ftm(ud->outputTypes.at(0));
+    tensor_t *b = nullptr;
+    tid_t made = _storage.create(step.outputs[0], om[0], _factory.get_tensor_size(om[0]), &b);
+    std::vector<tensor_t *> ins{_storage.get(step.inputs[0]), _storage.get(step.inputs[1])}, o{b};
+    ud_impl_t::tensor_args_t ta(ins), to(o);
+    ud->call_ud(params, ta, to);
+    ud_calls++;
+    if (idx != 1) _storage.remove_by_tid(lhs);
+    lhs = made;
+  }
+}
+
+// MOVE with several outputs = BROADCAST (command_runner.cc:57-84): on one GPU each
+// destination tid receives the whole tensor (header + payload) byte for byte.
+void command_runner_t::run_move(const command_t &cmd) {
+  tid_t src = cmd.inputs.at(0);
+  tensor_meta_t m = _storage.meta(src);
+  size_t bytes = _storage.bytes(src);
+  std::vector<tensor_t *> dsts;
+  for (tid_t d : cmd.outputs) {
+    if (d == src) continue;
+    tensor_t *b = nullptr;
+    _storage.create(d, m, bytes, &b);
+    dsts.push_back(b);
+  }
+  if (!dsts.empty()) check(bbts_hip_broadcast(_ctx, _storage.get(src), dsts.data(), (int)dsts.size(), _stream));
+  ud_calls++;
+}
+
+void command_runner_t::run_delete(const command_t &cmd) {
+  // the pool reuses freed blocks only for later work on the same stream
+  for (tid_t t : cmd.inputs) _storage.remove_by_tid(t);
+}
+
+float command_runner_t::predicted_cost(const command_t &cmd, bbts_costmodel *cm) {
+  // predicted microseconds: the cost model for matrix_mult (grouped-launch cost of its best
+  // tile), otherwise the 64-bit complexity hint at 1e-5 us per unit
+  if (cmd.type != command_t::APPLY) return 0.0f;
+  ud_impl_t *ud = _udm.get_fn_impl(cmd.fun_id);
+  if (!ud) return 0.0f;
+  std::vector<tensor_meta_t> in_meta(cmd.inputs.size());
+  std::vector<tensor_meta_t *> ip;
+  for (size_t i = 0; i < cmd.inputs.size(); ++i) {
+    in_meta[i] = _storage.meta(cmd.inputs[i]);
+    ip.push_back(&in_meta[i]);
+  }
+  if (ud->ud_name == "matrix_mult" && cm) {
+    const auto &a = in_meta[0].as<dense_tensor_meta_t>().m();
+    const auto &b = in_meta[1].as<dense_tensor_meta_t>().m();
+    const int nf = bbts_hip_costmodel_num_features(), nc = bbts_hip_num_tile_configs();
+    std::vector<float> f((size_t)(nf * nc)), y((size_t)nc);
+    for (int c = 0; c < nc; ++c)
+      bbts_hip_costmodel_features((int)a.num_rows, (int)b.num_cols, (int)a.num_cols, 0, BBTS_PREC_BF16X3, c,
+                                  &f[(size_t)(c * nf)]);
+    check(bbts_hip_costmodel_predict_host(cm, f.data(), nc, y.data()));
+    return std::exp(*std::min_element(y.begin(), y.end()));
+  }
+  ud_impl_t::tensor_params_t params;
+  params._params = cmd.params;
+  ud_impl_t::meta_args_t ia(ip);
+  return (float)ud->get_complexity_hint(params, ia) * 1e-5f;
+}
+
+void command_runner_t::run_matmul_batch(const std::vector<const command_t *> &cmds, int precision,
+                                        bbts_costmodel *cm) {
+  bbts_batch *batch = nullptr;
+  check(bbts_hip_batch_create(_ctx, &batch));
+  try {
+    for (const command_t *c : cmds) {
+      ud_impl_t *ud = _udm.get_fn_impl(c->fun_id);
+      ud_impl_t::tensor_params_t params;
+      params._params = c->params;
+      std::vector<tensor_t *> outs;
+      create_outputs(*c, ud, params, outs);
+      auto ma = _storage.meta(c->inputs[0]).as<dense_tensor_meta_t>().m();
+      auto mb = _storage.meta(c->inputs[1]).as<dense_tensor_meta_t>().m();
+      if (ma.num_cols != mb.num_rows) throw hip_error(BBTS_E_SHAPE, "matrix_mult shapes do not match");
+      bbts_gemm_desc_t d{};
+      d.M = (int32_t)ma.num_rows;
+      d.N = (int32_t)mb.num_cols;
+      d.K = (int32_t)ma.num_cols;
+      d.dtype = 0;
+      d.A = bbts_hip_block_data(_storage.get(c->inputs[0]));
+      d.B = bbts_hip_block_data(_storage.get(c->inputs[1]));
+      d.C = static_cast<float *>(bbts_hip_block_data(outs[0]));
+      d.lda = d.K;
+      d.ldb = d.N;
+      d.ldc = d.N;
+      check(bbts_hip_batch_add_gemm(batch, &d, precision));
+      batched_ops++;
+    }
+    check(bbts_hip_batch_plan(batch, cm ? BBTS_SCHED_LEARNED : BBTS_SCHED_HEURISTIC, cm));
+    check(bbts_hip_batch_launch(batch, _stream));
+    check(bbts_hip_sync(_ctx, _stream));
+    batches++;
+  } catch (...) {
+    bbts_hip_batch_destroy(batch);
+    throw;
+  }
+  bbts_hip_batch_destroy(batch);
+}
+
+void command_runner_t::run(reservation_station_t &rs, int policy, int precision, bbts_costmodel *cm) {
+  retire_order.clear();
+  const bool batched_ok = policy == BBTS_RT_BATCHED && precision != BBTS_PREC_FP32 &&
+                          precision != BBTS_PREC_FP8X1 && precision != BBTS_PREC_FP8X3;
+  while (!rs.empty()) {
+    auto &ready = rs.ready();
+    if (ready.empty()) throw std::runtime_error("no command is ready but commands are pending");
+    if (batched_ok) {
+      // batch-dequeue point: every ready matrix_mult APPLY without transposes goes to one grouped launch
+      std::vector<int32_t> now(ready.begin(), ready.end());
+      ready.clear();
+      std::vector<const command_t *> mm;
+      std::vector<int32_t> rest;
+      for (int32_t id : now) {
+        const command_t &c = rs.command(id);
+        ud_impl_t *ud = c.type == command_t::APPLY ? _udm.get_fn_impl(c.fun_id) : nullptr;
+        bool plain = c.params.size() < 1 || (!c.params[0].b && (c.params.size() < 2 || !c.params[1].b));
+        if (ud && ud->ud_name == "matrix_mult" && plain)
+          mm.push_back(&c);
+        else
+          rest.push_back(id);
+      }
+      if (!mm.empty()) run_matmul_batch(mm, precision, cm);
+      for (auto it = rest.rbegin(); it != rest.rend(); ++it) {
+        const command_t &c = rs.command(*it);
+        switch (c.type) {
+          case command_t::APPLY: run_apply(c, precision); break;
+          case command_t::REDUCE: run_reduce(c); break;
+          case command_t::MOVE: run_move(c); break;
+          case command_t::DELETE: run_delete(c); break;
+        }
+      }
+      for (const command_t *c : mm) {
+        retire_order.push_back(c->id);
+        rs.retire(c->id);
+        retired++;
+      }
+      for (auto it = rest.rbegin(); it != rest.rend(); ++it) {
+        retire_order.push_back(*it);
+        rs.retire(*it);
+        retired++;
+      }
+      continue;
+    }
+    int32_t id;
+    if (policy == BBTS_RT_SERIAL_LEARNED) {
+      // largest predicted work first among the ready commands
+      size_t best = 0;
+      float best_cost = -1.0f;
+      for (size_t i = 0; i < ready.size(); ++i) {
+        float c = predicted_cost(rs.command(ready[i]), cm);
+        if (c > best_cost) {
+          best = i;
+          best_cost = c;
+        }
+      }
+      id = ready[best];
+      ready.erase(ready.begin() + (long)best);
+    } else {
+      id = ready.back();  // reservation_station.cc:213-214: LIFO
+      ready.pop_back();
+    }
+    const command_t &c = rs.command(id);
+    switch (c.type) {
+      case command_t::APPLY: run_apply(c, precision); break;
+      case command_t::REDUCE: run_reduce(c); break;
+      case command_t::MOVE: run_move(c); break;
+      case command_t::DELETE: run_delete(c); break;
+    }
+    retire_order.push_back(id);
+    rs.retire(id);
+    retired++;
+  }
+  check(bbts_hip_sync(_ctx, _stream));
+}
+
+}  // namespace bbts
+
+// ============================================================================
+// C ABI
+// ============================================================================
+using namespace bbts;
+
+struct bbts_rt {
+  bbts_hip_ctx *ctx;
+  std::shared_ptr<tensor_factory_t> factory;
+  std::unique_ptr<udf_manager_t> udm;
+  std::unique_ptr<hbm_storage_t> storage;
+  reservation_station_t rs;
+  std::unique_ptr<command_runner_t> runner;
+};
+
+namespace {
+
+thread_local std::string g_rt_error;
+
+template <class F>
+int guarded(F &&f) {
+  try {
+    f();
+    return BBTS_OK;
+  } catch (const hip_error &e) {
+    g_rt_error = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    g_rt_error = e.what();
+    return BBTS_E_INVALID;
+  }
+}
+
+command_param_list_t to_params(const uint32_t *w, int n) {
+  command_param_list_t p((size_t)(n > 0 ? n : 0));
+  for (int i = 0; i < n; ++i) p[(size_t)i].u = w[i];
+  return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *bbts_rt_last_error(void) { return g_rt_error.c_str(); }
+
+int bbts_rt_create(bbts_hip_ctx *ctx, bbts_rt **out) {
+  if (!ctx || !out) return BBTS_E_INVALID;
+  return guarded([&] {
+    auto *rt = new bbts_rt();
+    rt->ctx = ctx;
+    rt->factory = std::make_shared<tensor_factory_t>();
+    rt->udm = std::make_unique<udf_manager_t>(rt->factory);
+    register_cutensor_library(*rt->factory, *rt->udm);
+    rt->storage = std::make_unique<hbm_storage_t>(ctx);
+    rt->runner = std::make_unique<command_runner_t>(*rt->storage, *rt->udm, *rt->factory, ctx);
+    *out = rt;
+  });
+}
+
+int bbts_rt_destroy(bbts_rt *rt) {
+  if (!rt) return BBTS_OK;
+  return guarded([&] {
+    bbts_hip_sync(rt->ctx, nullptr);
+    delete rt;
+  });
+}
+
+int bbts_rt_find_impl(bbts_rt *rt, const char *ud_name, int is_gpu, int32_t *ud_id, int32_t *impl_id) {
+  if (!rt || !ud_name || !ud_id || !impl_id) return BBTS_E_INVALID;
+  ud_func_t *f = rt->udm->get_udf(ud_name);
+  if (!f) return BBTS_E_INVALID;
+  ud_impl_t *impl = rt->udm->find_match(ud_name, f->impls.empty() ? std::vector<std::string>{}
+                                                                   : f->impls[0]->inputTypes,
+                                        f->impls.empty() ? std::vector<std::string>{} : f->impls[0]->outputTypes,
+                                        is_gpu != 0);
+  if (!impl) return BBTS_E_UNSUPPORTED;
+  *ud_id = impl->impl_id.ud_id;
+  *impl_id = impl->impl_id.impl_id;
+  return BBTS_OK;
+}
+
+int bbts_rt_format_id(bbts_rt *rt, const char *fmt_name, int32_t *fmt_id) {
+  if (!rt || !fmt_name || !fmt_id) return BBTS_E_INVALID;
+  *fmt_id = rt->factory->get_tensor_ftm(fmt_name);
+  return *fmt_id < 0 ? BBTS_E_FORMAT : BBTS_OK;
+}
+
+int bbts_rt_put_dense(bbts_rt *rt, int32_t tid, uint32_t rows, uint32_t cols, const float *host) {
+  if (!rt) return BBTS_E_INVALID;
+  return guarded([&] {
+    tensor_meta_t m;
+    std::memset(&m, 0, sizeof(m));
+    m.fmt_id = rt->factory->get_tensor_ftm("dense");
+    m.as<dense_tensor_meta_t>().m() = {rows, cols};
+    tensor_t *b = nullptr;
+    rt->storage->create(tid, m, rt->factory->get_tensor_size(m), &b);
+    if (host && rows && cols)
+      check(bbts_hip_block_upload(rt->ctx, b, host, (size_t)rows * cols * sizeof(float), nullptr));
+  });
+}
+
+int bbts_rt_put_cu(bbts_rt *rt, int32_t tid, int rank, const int64_t *dims, const float *host) {
+  if (!rt || rank < 0 || rank > 4 || (rank > 0 && !dims)) return BBTS_E_INVALID;
+  return guarded([&] {
+    tensor_meta_t m;
+    std::memset(&m, 0, sizeof(m));
+    m.fmt_id = rt->factory->get_tensor_ftm("cutensor");
+    auto &s = m.as<cu_meta_t>().m();
+    s.rank = (uint32_t)rank;
+    size_t n = 1;
+    for (int r = 0; r < rank; ++r) {
+      s.dims[r] = dims[r];
+      n *= (size_t)dims[r];
+    }
+    tensor_t *b = nullptr;
+    rt->storage->create(tid, m, rt->factory->get_tensor_size(m), &b);
+    if (host && n) check(bbts_hip_block_upload(rt->ctx, b, host, n * sizeof(float), nullptr));
+  });
+}
+
+int bbts_rt_has(bbts_rt *rt, int32_t tid) { return rt && rt->storage->has(tid) ? 1 : 0; }
+
+int bbts_rt_meta(bbts_rt *rt, int32_t tid, void *meta260, size_t *bytes) {
+  if (!rt) return BBTS_E_INVALID;
+  return guarded([&] {
+    tensor_meta_t m = rt->storage->meta(tid);
+    if (meta260) std::memcpy(meta260, &m, sizeof(m));
+    if (bytes) *bytes = rt->storage->bytes(tid);
+  });
+}
+
+int bbts_rt_get_bytes(bbts_rt *rt, int32_t tid, void *host, size_t bytes) {
+  if (!rt || !host) return BBTS_E_INVALID;
+  return guarded([&] { check(bbts_hip_block_download(rt->ctx, rt->storage->get(tid), host, bytes, nullptr)); });
+}
+
+int bbts_rt_num_tensors(bbts_rt *rt) { return rt ? (int)rt->storage->size() : 0; }
+
+static int queue(bbts_rt *rt, command_t &&c) {
+  return guarded([&] { rt->rs.queue_command(c, [&](tid_t t) { return rt->storage->has(t); }); });
+}
+
+int bbts_rt_apply(bbts_rt *rt, int32_t cmd_id, int32_t ud_id, int32_t impl_id, const int32_t *in, int nin,
+                  const int32_t *out, int nout, const uint32_t *params, int nparams) {
+  if (!rt || nin < 0 || nout < 1) return BBTS_E_INVALID;
+  command_t c;
+  c.id = cmd_id;
+  c.type = command_t::APPLY;
+  c.fun_id = {ud_id, impl_id};
+  c.inputs.assign(in, in + nin);
+  c.outputs.assign(out, out + nout);
+  c.params = to_params(params, nparams);
+  if (!rt->udm->get_fn_impl(c.fun_id)) return BBTS_E_INVALID;
+  return queue(rt, std::move(c));
+}
+
+int bbts_rt_reduce(bbts_rt *rt, int32_t cmd_id, int32_t ud_id, int32_t impl_id, const int32_t *in, int nin,
+                   int32_t out, const uint32_t *params, int nparams) {
+  if (!rt || nin < 1) return BBTS_E_INVALID;
+  command_t c;
+  c.id = cmd_id;
+  c.type = command_t::REDUCE;
+  c.fun_id = {ud_id, impl_id};
+  c.inputs.assign(in, in + nin);
+  c.outputs = {out};
+  c.params = to_params(params, nparams);
+  if (!rt->udm->get_fn_impl(c.fun_id)) return BBTS_E_INVALID;
+  return queue(rt, std::move(c));
+}
+
+int bbts_rt_broadcast(bbts_rt *rt, int32_t cmd_id, int32_t src, const int32_t *dst, int n) {
+  if (!rt || n < 1 || !dst) return BBTS_E_INVALID;
+  command_t c;
+  c.id = cmd_id;
+  c.type = command_t::MOVE;
+  c.inputs = {src};
+  c.outputs.assign(dst, dst + n);
+  return queue(rt, std::move(c));
+}
+
+int bbts_rt_delete(bbts_rt *rt, int32_t cmd_id, const int32_t *tids, int n) {
+  if (!rt || n < 1 || !tids) return BBTS_E_INVALID;
+  command_t c;
+  c.id = cmd_id;
+  c.type = command_t::DELETE;
+  c.inputs.assign(tids, tids + n);
+  return queue(rt, std::move(c));
+}
+
+int bbts_rt_run(bbts_rt *rt, int policy, int precision, bbts_costmodel *cm) {
+  if (!rt) return BBTS_E_INVALID;
+  return guarded([&] { rt->runner->run(rt->rs, policy, precision, cm); });
+}
+
+int bbts_rt_stats(bbts_rt *rt, int64_t *retired, int64_t *ud_calls, int64_t *batches, int64_t *batched_ops) {
+  if (!rt) return BBTS_E_INVALID;
+  if (retired) *retired = rt->runner->retired;
+  if (ud_calls) *ud_calls = rt->runner->ud_calls;
+  if (batches) *batches = rt->runner->batches;
+  if (batched_ops) *batched_ops = rt->runner->batched_ops;
+  return BBTS_OK;
+}
+
+int bbts_rt_retire_order(bbts_rt *rt, int32_t *ids, int cap, int *n) {
+  if (!rt || !n) return BBTS_E_INVALID;
+  int m = (int)rt->runner->retire_order.size();
+  for (int i = 0; i < m && i < cap; ++i) ids[i] = rt->runner->retire_order[(size_t)i];
+  *n = m;
+  return BBTS_OK;
+}
+
+}  // extern "C"

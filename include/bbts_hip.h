@@ -210,6 +210,8 @@ int bbts_hip_costmodel_features(int M, int N, int K, int dtype, int precision, i
 int bbts_hip_costmodel_train(bbts_costmodel *cm, const float *X, const float *y, int n, int batch, float lr,
                              int epochs, float *loss_out, bbts_stream_t stream);
 int bbts_hip_costmodel_predict(bbts_costmodel *cm, const float *X, int n, float *y, bbts_stream_t stream);
+/* The same on host arrays (staged through device memory; synchronous). */
+int bbts_hip_costmodel_predict_host(bbts_costmodel *cm, const float *X_host, int n, float *y_host);
 /* Forward + backward of the mean squared loss on one batch without updating, for gradient checks:
  * grads (device) receives the flattened parameter gradient, loss (device) the loss. */
 int bbts_hip_costmodel_grad(bbts_costmodel *cm, const float *X, const float *y, int n, float *grads, float *loss,

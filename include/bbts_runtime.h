@@ -1,0 +1,77 @@
+/*
+ * bbts_runtime.h -- C ABI of the host runtime mirror (libbbts_runtime.so).
+ *
+ * The C++ runtime behind it keeps the reference's per-node host interfaces on
+ * top of libbbts_hip.so: tensor_factory_t, ud_impl_t / udf_manager_t with the
+ * reference's registration order and impl ids, an HBM storage with the
+ * reference's tid semantics, a reservation station and a command runner
+ * (APPLY, local REDUCE, single-node BROADCAST, DELETE).  These functions let
+ * a caller without a C++ toolchain (Python, tests) queue commands and run them.
+ * All return 0 or a bbts_status code (bbts_hip.h); bbts_rt_last_error() has the message.
+ */
+#ifndef BBTS_RUNTIME_H
+#define BBTS_RUNTIME_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "bbts_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bbts_rt bbts_rt;
+
+/* Run policies of bbts_rt_run. */
+enum bbts_rt_policy {
+  BBTS_RT_SERIAL_LIFO = 0,   /* the reference's order: ready list popped last-in first-out
+                                (src/commands/reservation_station.cc:213-214), one ud call at a time */
+  BBTS_RT_BATCHED = 1,       /* every ready matrix_mult APPLY is drained into one grouped
+                                persistent launch (bbts_hip_batch_*), the rest run one at a time */
+  BBTS_RT_SERIAL_LEARNED = 2 /* one at a time, ready list ordered by the cost model (largest first) */
+};
+
+/* Message of the calling thread's last failed bbts_rt_* call (C++ exceptions of the runtime,
+ * the reference's error channel, become status codes here). */
+const char *bbts_rt_last_error(void);
+
+/* Node set-up (src/server/node.cc:9-54 minus MPI): factory, udf manager, storage, RS, runner. */
+int bbts_rt_create(bbts_hip_ctx *ctx, bbts_rt **rt);
+int bbts_rt_destroy(bbts_rt *rt);
+
+/* udf lookup: the impl a compiled command would carry (udf_manager.cc:10-37 matcher semantics). */
+int bbts_rt_find_impl(bbts_rt *rt, const char *ud_name, int is_gpu, int32_t *ud_id, int32_t *impl_id);
+/* Format id of a registered tensor format ("dense" = 0, "cutensor" = 1). */
+int bbts_rt_format_id(bbts_rt *rt, const char *fmt_name, int32_t *fmt_id);
+
+/* Tensors in storage (memory_storage.h:69-164). */
+int bbts_rt_put_dense(bbts_rt *rt, int32_t tid, uint32_t rows, uint32_t cols, const float *host);
+int bbts_rt_put_cu(bbts_rt *rt, int32_t tid, int rank, const int64_t *dims, const float *host);
+int bbts_rt_has(bbts_rt *rt, int32_t tid);
+int bbts_rt_meta(bbts_rt *rt, int32_t tid, void *meta260, size_t *bytes);
+/* Copies `bytes` of the tensor, starting at its header, to host memory. */
+int bbts_rt_get_bytes(bbts_rt *rt, int32_t tid, void *host, size_t bytes);
+int bbts_rt_num_tensors(bbts_rt *rt);
+
+/* Commands (src/commands/command.h:26-33).  Command ids must increase
+ * (reservation_station.cc:27-29).  params are the raw 32-bit words of command_param_t. */
+int bbts_rt_apply(bbts_rt *rt, int32_t cmd_id, int32_t ud_id, int32_t impl_id, const int32_t *in, int nin,
+                  const int32_t *out, int nout, const uint32_t *params, int nparams);
+int bbts_rt_reduce(bbts_rt *rt, int32_t cmd_id, int32_t ud_id, int32_t impl_id, const int32_t *in, int nin,
+                   int32_t out, const uint32_t *params, int nparams);
+int bbts_rt_broadcast(bbts_rt *rt, int32_t cmd_id, int32_t src, const int32_t *dst, int n);
+int bbts_rt_delete(bbts_rt *rt, int32_t cmd_id, const int32_t *tids, int n);
+
+/* Run every queued command; precision applies to matrix_mult (BBTS_PREC_*); cm may be NULL. */
+int bbts_rt_run(bbts_rt *rt, int policy, int precision, bbts_costmodel *cm);
+
+/* Counters: commands retired, ud calls issued, grouped launches, ops inside grouped launches. */
+int bbts_rt_stats(bbts_rt *rt, int64_t *retired, int64_t *ud_calls, int64_t *batches, int64_t *batched_ops);
+/* Order in which the last run retired commands (ids), up to cap entries; returns the count in *n. */
+int bbts_rt_retire_order(bbts_rt *rt, int32_t *ids, int cap, int *n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BBTS_RUNTIME_H */

@@ -68,3 +68,16 @@ def test_init_without_gpu_fails_loudly():
     h = ctypes.c_void_p()
     rc = bbts_amd.lib().bbts_hip_init(0, ctypes.byref(h))
     assert rc != 0
+
+
+def test_runtime_library_exports_every_declared_symbol():
+    _ensure_built()
+    from bbts_amd import runtime
+    text = open(os.path.join(ROOT, "include", "bbts_runtime.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    declared = sorted(set(re.findall(r"\b(bbts_rt_\w+)\s*\(", text)))
+    assert declared == runtime.declared_symbols()
+    out = subprocess.check_output(["nm", "-D", "--defined-only", runtime.RT_PATH], text=True)
+    exported = set(l.split()[-1] for l in out.splitlines() if " T " in l)
+    assert not [s for s in declared if s not in exported]
+    runtime.rtlib()

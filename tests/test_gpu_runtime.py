@@ -1,0 +1,146 @@
+"""The C++ runtime mirror end to end on the GPU: registration order / impl ids,
+the two-transaction APPLY protocol, local REDUCE, single-node BROADCAST,
+DELETE scheduling, and the three run policies, on a small bmm stream shaped
+like the reference's generate_bmm (applications/bmm/generate_bmm.cc:232-266).
+
+Bar: block metas byte-exact; uniform blocks bit-exact with the oracle
+generator; products <= 2e-6 relative Frobenius error vs FP64 (FP32 and
+BF16x6), reduce outputs equal to the oracle's left fold of the product blocks.
+"""
+import struct
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture()
+def rt(ctx):
+    from bbts_amd.runtime import Runtime
+    r = Runtime(ctx)
+    yield r
+    r.close()
+
+
+def test_registration_order_and_impl_ids(rt):
+    # udf_manager.cc:40-74 order; command_t carries these ids (command.h:466)
+    assert rt.impl("matrix_add", False) == (0, 0)
+    assert rt.impl("matrix_add", True) == (0, 1)
+    assert rt.impl("matrix_mult", False) == (1, 0)
+    assert rt.impl("matrix_mult", True) == (1, 1)
+    assert rt.impl("uniform", False) == (2, 0)
+    assert rt.impl("contraction", True) == (4, 0)
+    assert rt.format_id("dense") == 0 and rt.format_id("cutensor") == 1
+
+
+def _bmm_stream(rt, n_split, bs, seed, gpu=True):
+    """A = [a_ik], B = [b_kj] uniform blocks; C_ij = sum_k a_ik b_kj via APPLY + REDUCE; temps deleted."""
+    uni = rt.impl("uniform", False)
+    mult = rt.impl("matrix_mult", gpu)
+    add = rt.impl("matrix_add", gpu)
+    tid = lambda m, i, j: m * 10000 + i * 100 + j
+    for i in range(n_split):
+        for j in range(n_split):
+            rt.apply(uni, [], [tid(1, i, j)], [bs, bs, -1.0, 1.0, seed + tid(1, i, j)])
+            rt.apply(uni, [], [tid(2, i, j)], [bs, bs, -1.0, 1.0, seed + tid(2, i, j)])
+    temps = []
+    for i in range(n_split):
+        for j in range(n_split):
+            prods = []
+            for k in range(n_split):
+                t = tid(5, i * n_split + j, k)
+                rt.apply(mult, [tid(1, i, k), tid(2, k, j)], [t])
+                prods.append(t)
+            rt.reduce(add, prods, tid(3, i, j))
+            temps += prods
+    rt.delete(temps)
+    return tid
+
+
+def _check_bmm(rt, orc, n_split, bs, seed, tid, tol):
+    A = {(i, j): orc.uniform(bs * bs, seed + tid(1, i, j), -1.0, 1.0).reshape(bs, bs)
+         for i in range(n_split) for j in range(n_split)}
+    B = {(i, j): orc.uniform(bs * bs, seed + tid(2, i, j), -1.0, 1.0).reshape(bs, bs)
+         for i in range(n_split) for j in range(n_split)}
+    for (i, j), a in A.items():
+        assert np.array_equal(rt.get_dense(tid(1, i, j)), a)
+    for i in range(n_split):
+        for j in range(n_split):
+            ref = sum(orc.gemm_f64(A[i, k], B[k, j]) for k in range(n_split))
+            meta, nbytes = rt.meta(tid(3, i, j))
+            assert struct.unpack_from("<iII", meta, 0) == (0, bs, bs)
+            assert not any(meta[12:])
+            assert nbytes == 260 + 4 * bs * bs
+            got = rt.get_dense(tid(3, i, j)).astype(np.float64)
+            assert np.linalg.norm(got - ref) <= tol * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("policy,precision,gpu", [(0, 0, True), (0, 0, False), (2, 0, True), (1, 3, True)])
+def test_bmm_stream(rt, orc, policy, precision, gpu):
+    print("bmm stream policy=%d precision=%d" % (policy, precision), flush=True)
+    n_split, bs, seed = 3, 96, 1000
+    tid = _bmm_stream(rt, n_split, bs, seed, gpu)
+    rt.run(policy, precision)
+    _check_bmm(rt, orc, n_split, bs, seed, tid, 2e-6)
+    # only A, B and C remain: every product temp was deleted after its reduce
+    assert rt.num_tensors() == 3 * n_split * n_split
+    st = rt.stats()
+    assert st["retired"] == 2 * 9 + 27 + 9 + 1
+    if policy == 1:
+        assert st["batches"] >= 1 and st["batched_ops"] == 27
+
+
+def test_reduce_is_the_left_fold(rt, orc, ctx):
+    add = rt.impl("matrix_add", True)
+    xs = [orc.normal(40 * 50, 100 + q).reshape(40, 50) * (10.0 ** q) for q in range(5)]
+    for q, x in enumerate(xs):
+        rt.put_dense(10 + q, x)
+    rt.reduce(add, [10, 11, 12, 13, 14], 99)
+    rt.reduce(add, [10], 98)  # a single input is renamed (local_reduce_op.cc:27,90)
+    rt.run()
+    assert np.array_equal(rt.get_dense(99), orc.local_reduce(xs).reshape(40, 50))
+    assert rt.has(98) and not rt.has(10)
+
+
+def test_broadcast_command(rt, orc):
+    x = orc.normal(33 * 65, 7).reshape(33, 65)
+    rt.put_dense(1, x)
+    rt.broadcast(1, [2, 3, 4, 5])
+    rt.run()
+    src = rt.get_bytes(1)
+    for t in (2, 3, 4, 5):
+        assert np.array_equal(rt.get_bytes(t), src)
+
+
+def test_contraction_command(rt, orc):
+    dl, dr = [6, 5, 8, 4], [8, 4, 7, 3]
+    p = orc.contraction_params([0, 1, 2, 3], [2, 3, 4, 5], [0, 1, 4, 5], 2.0)
+    L = orc.normal(int(np.prod(dl)), 1)
+    R = orc.normal(int(np.prod(dr)), 2)
+    rt.put_cu(1, dl, L)
+    rt.put_cu(2, dr, R)
+    rt.apply(rt.impl("contraction", True), [1, 2], [3], [int(w) for w in p.view(np.uint32)])
+    rt.run()
+    meta, nbytes = rt.meta(3)
+    assert struct.unpack_from("<iI", meta, 0) == (1, 4)
+    assert struct.unpack_from("<4q", meta, 12) == (6, 5, 7, 3)
+    got = rt.get_bytes(3)[260:].view(np.float32).astype(np.float64)
+    ref, _ = orc.contraction(p, L, dl, R, dr, brute=True)
+    assert np.linalg.norm(got - ref) <= 2e-6 * np.linalg.norm(ref)
+
+
+def test_command_errors(rt):
+    from bbts_amd import BBTSError
+    from bbts_amd.runtime import rtlib
+    add = rt.impl("matrix_add", True)
+    with pytest.raises(BBTSError):
+        rt.apply(add, [123, 124], [125])  # inputs never created
+    rt.put_dense(1, np.zeros((2, 2), np.float32))
+    rt._next_cmd = 0
+    rt.apply(add, [1, 1], [2])
+    rt._next_cmd = 0  # the next id would not increase (reservation_station.cc:27-29)
+    with pytest.raises(BBTSError):
+        rt.apply(add, [1, 1], [3])
+    rt.run()
+    assert rt.has(2)
