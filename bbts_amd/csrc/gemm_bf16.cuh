@@ -3,14 +3,22 @@
 //
 // One workgroup = 256 lanes = 4 wave64s in a 2x2 grid computes a BM x BN tile
 // of C = alpha * A·B (row-major A: M x K, B: K x N, C: M x N, FP32 out).
-// Per K stage (BK = 32) both operands are staged into LDS as k-contiguous
-// rows of 32 BF16 (64 B), with 16-byte chunks XOR-swizzled by (row>>1)&3 so
-// that eight lanes reading one 16-byte chunk each from eight consecutive rows
-// hit eight distinct 16-byte bank groups.  B arrives N-contiguous and is
-// transposed in registers on the way in (two k rows x eight n per lane,
-// packed into 32-bit pairs).  MFMA: v_mfma_f32_32x32x16_bf16; each wave owns a
-// (BM/2) x (BN/2) sub-tile of 32x32 fragments.  Two LDS buffers: the next
-// stage's global loads are in flight while the current stage is consumed.
+// Per K stage (BK = 64 for one BF16 slice, 32 for split operands) both
+// operands are staged into LDS as k-contiguous rows of BK BF16 values, with
+// the 16-byte chunks of a row XOR-swizzled by the row index, so that eight
+// lanes reading one chunk each from eight consecutive rows (the MFMA operand
+// pattern) hit eight distinct 16-byte bank groups.  B arrives N-contiguous and
+// is transposed in registers on the way in: a lane holds two k rows x EPV n
+// columns and writes 32-bit (k, k+1) pairs; lanes are mapped k-pair fastest,
+// so the 32 lanes of a half-wave write 32 distinct banks of one LDS row.
+// MFMA: v_mfma_f32_32x32x16_bf16; each wave owns a (BM/2) x (BN/2) sub-tile of
+// 32x32 fragments.  Two LDS buffers: the next stage's global loads are in
+// flight while the current stage feeds the MFMAs; one barrier per stage.
+//
+// FAST tiles (the planner checks K, N and the leading dimensions are
+// multiples of the 16-byte vector and the operands 16-byte aligned) load
+// without branches: out-of-range vectors read a clamped address and are
+// zeroed by a select.  Other shapes use the element-wise bounds-checked path.
 //
 // Split precision (S slices, T terms): FP32 operands are split on the fly into
 // S BF16 slices x = x0 + x1 + x2 (x0 = bf16(x), x1 = bf16(x - x0), ...), and
@@ -27,11 +35,40 @@ namespace bf16gemm {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 
-constexpr int BK = 32;
 constexpr int kThreads = 256;
 
+// K per LDS stage: 64 for single-slice operands, 32 for split operands and for the
+// 256x256 tile (whose 256 accumulators leave no room for a 64-deep staging set)
+template <int BM, int BN, int S>
+struct stage_k {
+  static constexpr int value = (S == 1 && BM * BN <= 256 * 128) ? 64 : 32;
+};
+
+// element index of (row, k) in a [rows][BK] BF16 tile with swizzled 16-byte chunks
+template <int BK>
 __device__ __forceinline__ int sw(int row, int k) {
-  return row * BK + ((((k >> 3) ^ ((row >> 1) & 3))) << 3) + (k & 7);
+  const int x = BK == 64 ? (row & 7) : ((row >> 1) & 3);
+  return row * BK + ((((k >> 3) ^ x)) << 3) + (k & 7);
+}
+
+// element index of (k, n) in a [BK][BN] n-contiguous B tile; the 16-byte chunks of row k
+// are XOR-swizzled by 4*(k & 3), so the eight k rows one transposing read touches spread
+// over all bank groups
+template <int BN>
+__device__ __forceinline__ int swn(int k, int n) {
+  constexpr int CH = BN / 8;
+  return k * BN + ((((n >> 3) ^ (((k & 3) << 2) & (CH - 1)))) << 3) + (n & 7);
+}
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) v4s lds_v4s;
+
+// ds_read_b64_tr_b16: within each 16-lane group, source lane j addresses 4 consecutive
+// 16-bit elements; lane i receives element (i % 4) of source lanes 4e + i/4, e = 0..3.
+// With source lane j pointing at B[k0 + j/4][n0 + 4 (j % 4)], lane i receives
+// B[k0 .. k0+3][n0 + i]: four consecutive k of one column.
+__device__ __forceinline__ v4s read_tr16(const uint16_t *p) {
+  return __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s *)(p));
 }
 
 __device__ __forceinline__ uint16_t bf16_rne(float f) {
@@ -67,9 +104,8 @@ struct in_traits<float> {
 // LDS footprint (bytes) of a config.
 template <int BM, int BN, int S>
 struct smem_layout {
-  static constexpr int a_elems = BM * BK;
-  static constexpr int b_elems = BN * BK;
-  static constexpr int bytes = 2 * S * (a_elems + b_elems) * 2;
+  static constexpr int BK = stage_k<BM, BN, S>::value;
+  static constexpr int bytes = 2 * S * (BM + BN) * BK * 2;
 };
 
 struct op_view {
@@ -82,22 +118,29 @@ struct op_view {
   int vecA, vecB;
 };
 
+// Can op use the branch-free FAST path for operand type InT?
+inline bool fast_ok(const op_view &v, int epv) {
+  auto al = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
+  return al(v.A) && al(v.B) && v.K % epv == 0 && v.N % epv == 0 && v.lda % epv == 0 && v.ldb % epv == 0;
+}
+
 // Compute one BM x BN tile at (m0, n0).  smem must hold smem_layout<BM,BN,S>::bytes.
-template <int BM, int BN, typename InT, int S, int T>
+template <int BM, int BN, typename InT, int S, int T, bool FAST>
 __device__ __forceinline__ void tile(const op_view &op, int m0, int n0, uint16_t *smem) {
+  constexpr int BK = stage_k<BM, BN, S>::value;
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 32, FN = WN / 32;
   constexpr int AE = BM * BK, BE = BN * BK;
   constexpr int EPV = in_traits<InT>::kElemsPer16B;  // elements per 16-byte load
-  // A: BM rows x (BK / EPV) vectors per row
-  constexpr int A_VECS = BM * (BK / EPV);
+  constexpr int VPR = BK / EPV;                      // vectors per A row
+  constexpr int A_VECS = BM * VPR;
   constexpr int A_PER = (A_VECS + kThreads - 1) / kThreads;
-  // B: items of 2 k-rows x EPV n-columns
-  constexpr int B_ITEMS = (BK / 2) * (BN / EPV);
-  constexpr int B_PER = (B_ITEMS + kThreads - 1) / kThreads;
+  constexpr int BVPR = BN / EPV;                     // vectors per B row (n-contiguous)
+  constexpr int B_VECS = BK * BVPR;
+  constexpr int B_PER = (B_VECS + kThreads - 1) / kThreads;
 
-  uint16_t *As = smem;                      // [2][S][AE]
-  uint16_t *Bs = smem + 2 * S * AE;         // [2][S][BE]
+  uint16_t *As = smem;                // [2][S][BM][BK]  k-contiguous
+  uint16_t *Bs = smem + 2 * S * AE;   // [2][S][BK][BN]  n-contiguous
 
   const InT *__restrict__ A = static_cast<const InT *>(op.A);
   const InT *__restrict__ B = static_cast<const InT *>(op.B);
@@ -106,44 +149,52 @@ __device__ __forceinline__ void tile(const op_view &op, int m0, int n0, uint16_t
   const int wm = wave >> 1, wn = wave & 1;
 
   uint4 ra[A_PER];
-  uint4 rb[B_PER][2];
+  uint4 rb[B_PER];
 
   auto load = [&](int k0) {
 #pragma unroll
     for (int r = 0; r < A_PER; ++r) {
-      int q = tid + kThreads * r;
-      if (q < A_VECS) {
-        int row = q / (BK / EPV), c = q % (BK / EPV);
-        int gm = m0 + row, gk = k0 + c * EPV;
+      const int q = tid + kThreads * r;
+      if (A_VECS % kThreads == 0 || q < A_VECS) {
+        const int row = q / VPR, c = q % VPR;
+        const int gm = m0 + row, gk = k0 + c * EPV;
         const InT *p = A + (int64_t)gm * op.lda + gk;
-        if (op.vecA && gm < M && gk + EPV <= K) {
-          ra[r] = *reinterpret_cast<const uint4 *>(p);
+        if constexpr (FAST) {
+          const bool ok = gm < M && gk < K;
+          uint4 v = *reinterpret_cast<const uint4 *>(ok ? p : A);
+          ra[r] = ok ? v : uint4{0, 0, 0, 0};
         } else {
-          InT tmp[EPV];
+          if (op.vecA && gm < M && gk + EPV <= K) {
+            ra[r] = *reinterpret_cast<const uint4 *>(p);
+          } else {
+            InT tmp[EPV];
 #pragma unroll
-          for (int e = 0; e < EPV; ++e) tmp[e] = (gm < M && gk + e < K) ? p[e] : (InT)0;
-          __builtin_memcpy(&ra[r], tmp, 16);
+            for (int e = 0; e < EPV; ++e) tmp[e] = (gm < M && gk + e < K) ? p[e] : (InT)0;
+            __builtin_memcpy(&ra[r], tmp, 16);
+          }
         }
       }
     }
 #pragma unroll
     for (int r = 0; r < B_PER; ++r) {
-      int q = tid + kThreads * r;
-      if (q < B_ITEMS) {
-        // k-pair fastest across lanes: the transposing 32-bit LDS stores of one
-        // instruction then spread over the 16 banks of a row half (4-way, not 16-way)
-        int kp = q % (BK / 2), ng = q / (BK / 2);
-        int gk = k0 + kp * 2, gn = n0 + ng * EPV;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const InT *p = B + (int64_t)(gk + h) * op.ldb + gn;
-          if (op.vecB && gk + h < K && gn + EPV <= N) {
-            rb[r][h] = *reinterpret_cast<const uint4 *>(p);
+      const int q = tid + kThreads * r;
+      if (B_VECS % kThreads == 0 || q < B_VECS) {
+        // consecutive lanes read consecutive 16-byte pieces of one B row: coalesced
+        const int row = q / BVPR, c = q % BVPR;
+        const int gk = k0 + row, gn = n0 + c * EPV;
+        const InT *p = B + (int64_t)gk * op.ldb + gn;
+        if constexpr (FAST) {
+          const bool ok = gk < K && gn < N;
+          uint4 v = *reinterpret_cast<const uint4 *>(ok ? p : B);
+          rb[r] = ok ? v : uint4{0, 0, 0, 0};
+        } else {
+          if (op.vecB && gk < K && gn + EPV <= N) {
+            rb[r] = *reinterpret_cast<const uint4 *>(p);
           } else {
             InT tmp[EPV];
 #pragma unroll
-            for (int e = 0; e < EPV; ++e) tmp[e] = (gk + h < K && gn + e < N) ? p[e] : (InT)0;
-            __builtin_memcpy(&rb[r][h], tmp, 16);
+            for (int e = 0; e < EPV; ++e) tmp[e] = (gk < K && gn + e < N) ? p[e] : (InT)0;
+            __builtin_memcpy(&rb[r], tmp, 16);
           }
         }
       }
@@ -155,11 +206,11 @@ __device__ __forceinline__ void tile(const op_view &op, int m0, int n0, uint16_t
     uint16_t *bs = Bs + buf * S * BE;
 #pragma unroll
     for (int r = 0; r < A_PER; ++r) {
-      int q = tid + kThreads * r;
-      if (q < A_VECS) {
-        int row = q / (BK / EPV), c = q % (BK / EPV);
+      const int q = tid + kThreads * r;
+      if (A_VECS % kThreads == 0 || q < A_VECS) {
+        const int row = q / VPR, c = q % VPR;
         if constexpr (EPV == 8) {
-          *reinterpret_cast<uint4 *>(&as[sw(row, c * 8)]) = ra[r];
+          *reinterpret_cast<uint4 *>(&as[sw<BK>(row, c * 8)]) = ra[r];
         } else {
           const float *f = reinterpret_cast<const float *>(&ra[r]);
           uint16_t sl[4][S];
@@ -170,34 +221,29 @@ __device__ __forceinline__ void tile(const op_view &op, int m0, int n0, uint16_t
             uint2 v;
             v.x = (uint32_t)sl[0][s] | ((uint32_t)sl[1][s] << 16);
             v.y = (uint32_t)sl[2][s] | ((uint32_t)sl[3][s] << 16);
-            *reinterpret_cast<uint2 *>(&as[s * AE + sw(row, c * 4)]) = v;
+            *reinterpret_cast<uint2 *>(&as[s * AE + sw<BK>(row, c * 4)]) = v;
           }
         }
       }
     }
 #pragma unroll
     for (int r = 0; r < B_PER; ++r) {
-      int q = tid + kThreads * r;
-      if (q < B_ITEMS) {
-        int kp = q % (BK / 2), ng = q / (BK / 2);
-        int k = kp * 2, n = ng * EPV;
+      const int q = tid + kThreads * r;
+      if (B_VECS % kThreads == 0 || q < B_VECS) {
+        const int row = q / BVPR, c = q % BVPR;
         if constexpr (EPV == 8) {
-          const uint16_t *h0 = reinterpret_cast<const uint16_t *>(&rb[r][0]);
-          const uint16_t *h1 = reinterpret_cast<const uint16_t *>(&rb[r][1]);
-#pragma unroll
-          for (int e = 0; e < 8; ++e)
-            *reinterpret_cast<uint32_t *>(&bs[sw(n + e, k)]) = (uint32_t)h0[e] | ((uint32_t)h1[e] << 16);
+          *reinterpret_cast<uint4 *>(&bs[swn<BN>(row, c * 8)]) = rb[r];
         } else {
-          const float *f0 = reinterpret_cast<const float *>(&rb[r][0]);
-          const float *f1 = reinterpret_cast<const float *>(&rb[r][1]);
+          const float *f = reinterpret_cast<const float *>(&rb[r]);
+          uint16_t sl[4][S];
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            uint16_t s0[S], s1[S];
-            split<S>(f0[e], s0);
-            split<S>(f1[e], s1);
+          for (int e = 0; e < 4; ++e) split<S>(f[e], sl[e]);
 #pragma unroll
-            for (int s = 0; s < S; ++s)
-              *reinterpret_cast<uint32_t *>(&bs[s * BE + sw(n + e, k)]) = (uint32_t)s0[s] | ((uint32_t)s1[s] << 16);
+          for (int s = 0; s < S; ++s) {
+            uint2 v;
+            v.x = (uint32_t)sl[0][s] | ((uint32_t)sl[1][s] << 16);
+            v.y = (uint32_t)sl[2][s] | ((uint32_t)sl[3][s] << 16);
+            *reinterpret_cast<uint2 *>(&bs[s * BE + swn<BN>(row, c * 4)]) = v;
           }
         }
       }
@@ -234,13 +280,22 @@ __device__ __forceinline__ void tile(const op_view &op, int m0, int n0, uint16_t
       for (int s = 0; s < S; ++s) {
 #pragma unroll
         for (int i = 0; i < FM; ++i) {
-          uint4 v = *reinterpret_cast<const uint4 *>(&as[s * AE + sw(wm * WM + i * 32 + col, k)]);
+          uint4 v = *reinterpret_cast<const uint4 *>(&as[s * AE + sw<BK>(wm * WM + i * 32 + col, k)]);
           af[s][i] = __builtin_bit_cast(bf16x8, v);
         }
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
-          uint4 v = *reinterpret_cast<const uint4 *>(&bs[s * BE + sw(wn * WN + j * 32 + col, k)]);
-          bfr[s][j] = __builtin_bit_cast(bf16x8, v);
+          // MFMA B operand of lane l: B[kk*16 + 8*(l/32) + t][n = l % 32], t = 0..7, read as
+          // two transposing 4-row reads of the 16-lane group g = l/16
+          const int g = lane >> 4, i = lane & 15;
+          const int kr = kk * 16 + 8 * (g >> 1) + (i >> 2);
+          const int nc = wn * WN + j * 32 + 16 * (g & 1) + 4 * (i & 3);
+          v4s lo = read_tr16(&bs[s * BE + swn<BN>(kr, nc)]);
+          v4s hi = read_tr16(&bs[s * BE + swn<BN>(kr + 4, nc)]);
+          struct {
+            v4s a, b;
+          } both{lo, hi};
+          bfr[s][j] = __builtin_bit_cast(bf16x8, both);
         }
       }
       // small terms first, the leading product last

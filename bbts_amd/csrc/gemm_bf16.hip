@@ -12,6 +12,8 @@
 // node and no host work.
 #include "gemm_bf16.cuh"
 
+#include <tuple>
+
 namespace bbts_hip {
 
 using namespace bf16gemm;
@@ -40,7 +42,7 @@ __device__ inline void map_tile2(int lin, int tiles_m, int tiles_n, int *tm, int
   *tn = in_group / gsize;
 }
 
-template <int BM, int BN, typename InT, int S, int T>
+template <int BM, int BN, typename InT, int S, int T, bool FAST>
 __global__ __launch_bounds__(256) void gemm_single_kernel(op_view op, int64_t sa, int64_t sb, int64_t sc, int tiles_m,
                                                           int tiles_n, uint32_t *meta_dst, uint32_t w0, uint32_t w1) {
   extern __shared__ __align__(16) uint16_t smem[];
@@ -54,10 +56,10 @@ __global__ __launch_bounds__(256) void gemm_single_kernel(op_view op, int64_t sa
   o.A = static_cast<const InT *>(op.A) + (int64_t)blockIdx.z * sa;
   o.B = static_cast<const InT *>(op.B) + (int64_t)blockIdx.z * sb;
   o.C = op.C + (int64_t)blockIdx.z * sc;
-  tile<BM, BN, InT, S, T>(o, tm * BM, tn * BN, smem);
+  tile<BM, BN, InT, S, T, FAST>(o, tm * BM, tn * BN, smem);
 }
 
-template <int BM, int BN, typename InT, int S, int T>
+template <int BM, int BN, typename InT, int S, int T, bool FAST>
 __global__ __launch_bounds__(256) void gemm_grouped_kernel(const gop *__restrict__ ops, const int2 *__restrict__ items,
                                                            int nitems, unsigned *counter) {
   extern __shared__ __align__(16) uint16_t smem[];
@@ -74,7 +76,7 @@ __global__ __launch_bounds__(256) void gemm_grouped_kernel(const gop *__restrict
     const int2 it = items[t];
     const gop o = ops[it.x];
     const int tm = it.y / o.tiles_n, tn = it.y % o.tiles_n;
-    tile<BM, BN, InT, S, T>(o.v, tm * BM, tn * BN, smem);
+    tile<BM, BN, InT, S, T, FAST>(o.v, tm * BM, tn * BN, smem);
   }
 }
 
@@ -89,49 +91,63 @@ constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 typedef void (*single_fn)(op_view, int64_t, int64_t, int64_t, int, int, uint32_t *, uint32_t, uint32_t);
 typedef void (*grouped_fn)(const gop *, const int2 *, int, unsigned *);
 
-template <int BM, int BN>
+template <int BM, int BN, bool FAST>
 struct cfg_kernels {
   static single_fn single(int variant) {
     switch (variant) {
-      case 0: return gemm_single_kernel<BM, BN, uint16_t, 1, 1>;
-      case 1: return gemm_single_kernel<BM, BN, float, 1, 1>;
-      case 2: return gemm_single_kernel<BM, BN, float, 2, 2>;
-      default: return gemm_single_kernel<BM, BN, float, 3, 3>;
+      case 0: return gemm_single_kernel<BM, BN, uint16_t, 1, 1, FAST>;
+      case 1: return gemm_single_kernel<BM, BN, float, 1, 1, FAST>;
+      case 2: return gemm_single_kernel<BM, BN, float, 2, 2, FAST>;
+      default: return gemm_single_kernel<BM, BN, float, 3, 3, FAST>;
     }
   }
   static grouped_fn grouped(int variant) {
     switch (variant) {
-      case 0: return gemm_grouped_kernel<BM, BN, uint16_t, 1, 1>;
-      case 1: return gemm_grouped_kernel<BM, BN, float, 1, 1>;
-      case 2: return gemm_grouped_kernel<BM, BN, float, 2, 2>;
-      default: return gemm_grouped_kernel<BM, BN, float, 3, 3>;
+      case 0: return gemm_grouped_kernel<BM, BN, uint16_t, 1, 1, FAST>;
+      case 1: return gemm_grouped_kernel<BM, BN, float, 1, 1, FAST>;
+      case 2: return gemm_grouped_kernel<BM, BN, float, 2, 2, FAST>;
+      default: return gemm_grouped_kernel<BM, BN, float, 3, 3, FAST>;
     }
   }
 };
 
-int smem_bytes(int cfg, int variant) {
-  int S = variant == 0 ? 1 : variant;
-  return 2 * S * (kCfgs[cfg].bm + kCfgs[cfg].bn) * BK * 2;
+int stage_k_of(int cfg, int variant) {
+  return (variant <= 1 && kCfgs[cfg].bm * kCfgs[cfg].bn <= 256 * 128) ? 64 : 32;
 }
 
-single_fn get_single(int cfg, int variant) {
+int smem_bytes(int cfg, int variant) {
+  int S = variant == 0 ? 1 : variant;
+  return 2 * S * (kCfgs[cfg].bm + kCfgs[cfg].bn) * stage_k_of(cfg, variant) * 2;
+}
+
+// a config fits when its two LDS stages fit the 160 KiB of a CU
+bool valid_cfg(int cfg, int variant) { return smem_bytes(cfg, variant) <= 160 * 1024; }
+
+// the element-wise (non-FAST) path is built for the 128x128 and 64x64 tiles only
+int generic_cfg(int cfg) { return (kCfgs[cfg].bm >= 128 && kCfgs[cfg].bn >= 128) ? 1 : 3; }
+
+single_fn get_single(int cfg, int variant, bool fast) {
+  if (!fast) return cfg == 1 ? cfg_kernels<128, 128, false>::single(variant)
+                             : cfg_kernels<64, 64, false>::single(variant);
   switch (cfg) {
-    case 0: return cfg_kernels<256, 128>::single(variant);
-    case 1: return cfg_kernels<128, 128>::single(variant);
-    case 2: return cfg_kernels<128, 64>::single(variant);
-    case 3: return cfg_kernels<64, 64>::single(variant);
-    case 4: return cfg_kernels<64, 128>::single(variant);
-    default: return cfg_kernels<256, 256>::single(variant);
+    case 0: return cfg_kernels<256, 128, true>::single(variant);
+    case 1: return cfg_kernels<128, 128, true>::single(variant);
+    case 2: return cfg_kernels<128, 64, true>::single(variant);
+    case 3: return cfg_kernels<64, 64, true>::single(variant);
+    case 4: return cfg_kernels<64, 128, true>::single(variant);
+    default: return cfg_kernels<256, 256, true>::single(variant);
   }
 }
-grouped_fn get_grouped(int cfg, int variant) {
+grouped_fn get_grouped(int cfg, int variant, bool fast) {
+  if (!fast) return cfg == 1 ? cfg_kernels<128, 128, false>::grouped(variant)
+                             : cfg_kernels<64, 64, false>::grouped(variant);
   switch (cfg) {
-    case 0: return cfg_kernels<256, 128>::grouped(variant);
-    case 1: return cfg_kernels<128, 128>::grouped(variant);
-    case 2: return cfg_kernels<128, 64>::grouped(variant);
-    case 3: return cfg_kernels<64, 64>::grouped(variant);
-    case 4: return cfg_kernels<64, 128>::grouped(variant);
-    default: return cfg_kernels<256, 256>::grouped(variant);
+    case 0: return cfg_kernels<256, 128, true>::grouped(variant);
+    case 1: return cfg_kernels<128, 128, true>::grouped(variant);
+    case 2: return cfg_kernels<128, 64, true>::grouped(variant);
+    case 3: return cfg_kernels<64, 64, true>::grouped(variant);
+    case 4: return cfg_kernels<64, 128, true>::grouped(variant);
+    default: return cfg_kernels<256, 256, true>::grouped(variant);
   }
 }
 
@@ -230,8 +246,11 @@ int launch_gemm_split_bf16(bbts_hip_ctx *ctx, const gemm_args &g, int slices_a, 
   v.M = g.M; v.N = g.N; v.K = g.K; v.alpha = g.alpha;
   v.vecA = al16(A) && lda % 4 == 0 && (g.batch == 1 || sa % 4 == 0);
   v.vecB = al16(B) && ldb % 4 == 0 && (g.batch == 1 || sb % 4 == 0);
+  const bool fast = fast_ok(v, 4) && (g.batch == 1 || (sa % 4 == 0 && sb % 4 == 0));
+  if (!fast) cfg = generic_cfg(cfg);
+  if (!valid_cfg(cfg, variant)) cfg = 1;
   int tiles_m = (g.M + kCfgs[cfg].bm - 1) / kCfgs[cfg].bm, tiles_n = (g.N + kCfgs[cfg].bn - 1) / kCfgs[cfg].bn;
-  single_fn fn = get_single(cfg, variant);
+  single_fn fn = get_single(cfg, variant, fast);
   int bytes = smem_bytes(cfg, variant);
   if ((rc = ensure_smem((const void *)fn, bytes))) return rc;
   hipLaunchKernelGGL(fn, dim3((unsigned)(tiles_m * tiles_n), 1, (unsigned)g.batch), dim3(256), (unsigned)bytes, s, v,
@@ -261,6 +280,7 @@ struct bbts_batch {
   std::vector<float> tile_us;  // planned per-tile cost per op (ordering key)
   struct group {
     int cfg = 0, variant = 0;
+    bool fast = true;
     int nitems = 0, grid = 0;
     double flops = 0.0;
     size_t ops_off = 0, items_off = 0, counter_off = 0;
@@ -271,6 +291,10 @@ struct bbts_batch {
   bool planned = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  std::vector<int> order;  // groups by descending work
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> join;
+  hipEvent_t fork = nullptr;
 };
 
 namespace bbts_hip {
@@ -312,7 +336,7 @@ int bbts_hip_tile_config_shape(int cfg, int *bm, int *bn, int *bk) {
   BBTS_REQUIRE(cfg >= 0 && cfg < kNumCfgs, BBTS_E_INVALID, "config %d out of range", cfg);
   if (bm) *bm = kCfgs[cfg].bm;
   if (bn) *bn = kCfgs[cfg].bn;
-  if (bk) *bk = BK;
+  if (bk) *bk = 64;
   return BBTS_OK;
 }
 
@@ -326,8 +350,11 @@ int bbts_hip_gemm_desc_launch(bbts_hip_ctx *ctx, const bbts_gemm_desc_t *d, int 
   int variant;
   if ((rc = variant_of(d->dtype, precision, &variant))) return rc;
   op_view v = view_of(*d);
+  BBTS_REQUIRE(valid_cfg(cfg, variant), BBTS_E_UNSUPPORTED, "config %d does not fit LDS for this precision", cfg);
+  const bool fast = fast_ok(v, d->dtype == 1 ? 8 : 4);
+  if (!fast) cfg = generic_cfg(cfg);
   int tiles_m = (d->M + kCfgs[cfg].bm - 1) / kCfgs[cfg].bm, tiles_n = (d->N + kCfgs[cfg].bn - 1) / kCfgs[cfg].bn;
-  single_fn fn = get_single(cfg, variant);
+  single_fn fn = get_single(cfg, variant, fast);
   int bytes = smem_bytes(cfg, variant);
   if ((rc = ensure_smem((const void *)fn, bytes))) return rc;
   hipLaunchKernelGGL(fn, dim3((unsigned)(tiles_m * tiles_n)), dim3(256), (unsigned)bytes, (hipStream_t)stream, v,
@@ -362,6 +389,9 @@ int bbts_hip_batch_destroy(bbts_batch *b) {
   (void)hipDeviceSynchronize();
   drop_graph(b);
   if (b->dev) (void)hipFree(b->dev);
+  for (auto s : b->side) (void)hipStreamDestroy(s);
+  for (auto e : b->join) (void)hipEventDestroy(e);
+  if (b->fork) (void)hipEventDestroy(b->fork);
   delete b;
   return BBTS_OK;
 }
@@ -442,9 +472,14 @@ int bbts_hip_batch_plan(bbts_batch *b, int policy, bbts_costmodel *cm) {
     for (size_t i = 0; i < n; ++i) {
       int best = 0;
       float best_us = 0;
+      int variant = 0;
+      variant_of(b->ops[i].dtype, b->prec[i], &variant);
+      bool first = true;
       for (int c = 0; c < kNumCfgs; ++c) {
+        if (!valid_cfg(c, variant)) continue;
         float us = expf(pred[i * kNumCfgs + (size_t)c]);  // predicted grouped cost of the op, microseconds
-        if (c == 0 || us < best_us) {
+        if (first || us < best_us) {
+          first = false;
           best = c;
           best_us = us;
         }
@@ -457,14 +492,17 @@ int bbts_hip_batch_plan(bbts_batch *b, int policy, bbts_costmodel *cm) {
     return BBTS_E_INVALID;
   }
 
-  // group ops by (cfg, variant); order the ops of a group by tile cost, largest first
-  std::map<std::pair<int, int>, std::vector<int>> gmap;
+  // group ops by (cfg, variant, fast); order the ops of a group by tile cost, largest first
+  std::map<std::tuple<int, int, int>, std::vector<int>> gmap;
   for (size_t i = 0; i < n; ++i) {
     const auto &d = b->ops[i];
     if (d.M == 0 || d.N == 0) continue;
     int variant;
     if ((rc = variant_of(d.dtype, b->prec[i], &variant))) return rc;
-    gmap[{b->cfg[i], variant}].push_back((int)i);
+    if (!valid_cfg(b->cfg[i], variant)) b->cfg[i] = 1;
+    const bool fast = fast_ok(view_of(d), d.dtype == 1 ? 8 : 4);
+    if (!fast) b->cfg[i] = generic_cfg(b->cfg[i]);
+    gmap[std::make_tuple(b->cfg[i], variant, fast ? 1 : 0)].push_back((int)i);
   }
   b->groups.clear();
   size_t bytes = 0;
@@ -479,8 +517,9 @@ int bbts_hip_batch_plan(bbts_batch *b, int policy, bbts_costmodel *cm) {
       });
     }
     bbts_batch::group grp;
-    grp.cfg = kv.first.first;
-    grp.variant = kv.first.second;
+    grp.cfg = std::get<0>(kv.first);
+    grp.variant = std::get<1>(kv.first);
+    grp.fast = std::get<2>(kv.first) != 0;
     std::vector<gop> ops;
     std::vector<int2> items;
     for (size_t q = 0; q < idx.size(); ++q) {
@@ -529,6 +568,10 @@ int bbts_hip_batch_plan(bbts_batch *b, int policy, bbts_costmodel *cm) {
     }
     BBTS_HIP_CHECK(hipMemcpy(b->dev, host.data(), bytes, hipMemcpyHostToDevice));
   }
+  b->order.resize(b->groups.size());
+  for (size_t g = 0; g < b->groups.size(); ++g) b->order[g] = (int)g;
+  std::stable_sort(b->order.begin(), b->order.end(),
+                   [&](int x, int y) { return b->groups[(size_t)x].flops > b->groups[(size_t)y].flops; });
   b->planned = true;
   return BBTS_OK;
 }
@@ -539,7 +582,7 @@ int bbts_hip_batch_launch_group(bbts_batch *b, int g, bbts_stream_t stream) {
   if (rc) return rc;
   const auto &grp = b->groups[(size_t)g];
   char *base = static_cast<char *>(b->dev);
-  grouped_fn fn = get_grouped(grp.cfg, grp.variant);
+  grouped_fn fn = get_grouped(grp.cfg, grp.variant, grp.fast);
   int bytes = smem_bytes(grp.cfg, grp.variant);
   if ((rc = ensure_smem((const void *)fn, bytes))) return rc;
   hipLaunchKernelGGL(fn, dim3((unsigned)grp.grid), dim3(256), (unsigned)bytes, (hipStream_t)stream,
@@ -580,7 +623,8 @@ int bbts_hip_batch_capture(bbts_batch *b, bbts_stream_t stream) {
   drop_graph(b);
   // kernel attributes must be set outside capture
   for (auto &grp : b->groups)
-    if ((rc = ensure_smem((const void *)get_grouped(grp.cfg, grp.variant), smem_bytes(grp.cfg, grp.variant))))
+    if ((rc = ensure_smem((const void *)get_grouped(grp.cfg, grp.variant, grp.fast),
+                          smem_bytes(grp.cfg, grp.variant))))
       return rc;
   hipStream_t s = (hipStream_t)stream;
   hipStream_t cap = s;
@@ -589,8 +633,39 @@ int bbts_hip_batch_capture(bbts_batch *b, bbts_stream_t stream) {
     BBTS_HIP_CHECK(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
     cap = own;
   }
+  // one side stream + join event per extra group: the graph launches the groups
+  // concurrently (fork from the capture stream, join back), so the persistent grid of one
+  // tile configuration fills the CUs another group's tail leaves idle
+  const int ng = (int)b->groups.size();
+  while ((int)b->side.size() < ng - 1) {
+    hipStream_t ss;
+    hipEvent_t ev;
+    BBTS_HIP_CHECK(hipStreamCreateWithFlags(&ss, hipStreamNonBlocking));
+    BBTS_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    b->side.push_back(ss);
+    b->join.push_back(ev);
+  }
+  if (!b->fork) BBTS_HIP_CHECK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
   BBTS_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-  rc = bbts_hip_batch_launch(b, cap);
+  if (ng > 1) {
+    // largest group on the capture stream, the others on side streams
+    hipError_t e0 = hipEventRecord(b->fork, cap);
+    for (int g = 1; g < ng && rc == BBTS_OK && e0 == hipSuccess; ++g) {
+      hipStream_t ss = b->side[(size_t)(g - 1)];
+      e0 = hipStreamWaitEvent(ss, b->fork, 0);
+      if (e0 == hipSuccess) rc = bbts_hip_batch_launch_group(b, b->order[(size_t)g], ss);
+      if (e0 == hipSuccess && rc == BBTS_OK) e0 = hipEventRecord(b->join[(size_t)(g - 1)], ss);
+    }
+    if (rc == BBTS_OK && e0 == hipSuccess) rc = bbts_hip_batch_launch_group(b, b->order[0], cap);
+    for (int g = 1; g < ng && rc == BBTS_OK && e0 == hipSuccess; ++g)
+      e0 = hipStreamWaitEvent(cap, b->join[(size_t)(g - 1)], 0);
+    if (e0 != hipSuccess && rc == BBTS_OK) {
+      set_error("graph fork/join failed: %s", hipGetErrorString(e0));
+      rc = BBTS_E_HIP;
+    }
+  } else {
+    rc = bbts_hip_batch_launch(b, cap);
+  }
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(cap, &graph);
   if (own) (void)hipStreamDestroy(own);
@@ -664,7 +739,10 @@ int bbts_hip_tune_batch(bbts_batch *b, int reps, float *times_us) {
     if ((rc = variant_of(d.dtype, prec, &variant))) break;
     int best = -1;
     float best_us = 0.0f;
+    const bool fast = fast_ok(view_of(d), d.dtype == 1 ? 8 : 4);
     for (int c = 0; c < kNumCfgs; ++c) {
+      // configs that do not fit LDS, or that the element-wise path maps elsewhere, are not timed
+      if (!valid_cfg(c, variant) || (!fast && generic_cfg(c) != c)) continue;
       long long tiles = tiles_of(d, c);
       long long target = 4LL * b->ctx->num_cus * occupancy(c, variant);
       long long R = (target + tiles - 1) / tiles;

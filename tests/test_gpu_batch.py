@@ -107,10 +107,13 @@ def test_tuner_fills_table(ctx, orc, torch_cuda):
     for (M, N, K, a, bb, ta, tb, C) in ops:
         b.add(M, N, K, ta.data_ptr(), tb.data_ptr(), C.data_ptr(), dtype=1, precision=1)
     t = b.tune(reps=2)
-    assert t.shape == (4, 6) and np.all(t > 0)
+    # configs that do not apply to an op (LDS too small, or an op outside the branch-free
+    # path such as K = 70, which only runs the element-wise 128x128 / 64x64 tiles) are -1
+    assert t.shape == (4, 6) and np.all((t > 0).any(axis=1))
+    assert np.all(t[2, [0, 2, 4, 5]] < 0) and np.all(t[2, [1, 3]] > 0)
     b.plan(2)
     for i in range(4):
-        assert b.op_config(i) == int(np.argmin(t[i]))
+        assert b.op_config(i) == int(np.argmin(np.where(t[i] > 0, t[i], np.inf)))
     b.launch()
     ctx.sync()
     _check(orc, ops, 2e-6)

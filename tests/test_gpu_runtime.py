@@ -96,11 +96,13 @@ def test_reduce_is_the_left_fold(rt, orc, ctx):
     xs = [orc.normal(40 * 50, 100 + q).reshape(40, 50) * (10.0 ** q) for q in range(5)]
     for q, x in enumerate(xs):
         rt.put_dense(10 + q, x)
+    rt.put_dense(20, xs[0])
     rt.reduce(add, [10, 11, 12, 13, 14], 99)
-    rt.reduce(add, [10], 98)  # a single input is renamed (local_reduce_op.cc:27,90)
+    rt.reduce(add, [20], 98)  # a single input is renamed (local_reduce_op.cc:27,90)
     rt.run()
     assert np.array_equal(rt.get_dense(99), orc.local_reduce(xs).reshape(40, 50))
-    assert rt.has(98) and not rt.has(10)
+    assert rt.has(98) and not rt.has(20)
+    assert np.array_equal(rt.get_dense(98), xs[0])
 
 
 def test_broadcast_command(rt, orc):
