@@ -120,6 +120,7 @@ struct tune_key {
 struct tune_val {
   int cfg;
   float us;
+  std::vector<float> all;  // microseconds per tile config, -1 where not timed
 };
 
 }  // namespace bbts_hip

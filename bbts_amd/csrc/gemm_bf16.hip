@@ -59,21 +59,28 @@ __global__ __launch_bounds__(256) void gemm_single_kernel(op_view op, int64_t sa
   tile<BM, BN, InT, S, T, FAST>(o, tm * BM, tn * BN, smem);
 }
 
+// Queue x of nq holds items [qstart[x], qstart[x+1]); workgroup w serves queue w % nq
+// (the grid is a multiple of nq), with its own ticket counter at counters[64 x].
 template <int BM, int BN, typename InT, int S, int T, bool FAST>
 __global__ __launch_bounds__(256) void gemm_grouped_kernel(const gop *__restrict__ ops, const int2 *__restrict__ items,
-                                                           int nitems, unsigned *counter) {
+                                                           const int *__restrict__ qstart, int nq,
+                                                           unsigned *counters) {
   extern __shared__ __align__(16) uint16_t smem[];
   __shared__ int s_ticket;
+  const int x = (int)blockIdx.x % nq;
+  const int wgs = (int)gridDim.x / nq;
+  const int base = qstart[x], len = qstart[x + 1] - base;
+  unsigned *counter = counters + 64 * x;
   for (;;) {
     if (threadIdx.x == 0) s_ticket = (int)atomicAdd(counter, 1u);
     __syncthreads();
     const int t = s_ticket;
     __syncthreads();
-    if (t >= nitems) {
-      if (threadIdx.x == 0 && t == nitems + (int)gridDim.x - 1) atomicExch(counter, 0u);
+    if (t >= len) {
+      if (threadIdx.x == 0 && t == len + wgs - 1) atomicExch(counter, 0u);
       return;
     }
-    const int2 it = items[t];
+    const int2 it = items[base + t];
     const gop o = ops[it.x];
     const int tm = it.y / o.tiles_n, tn = it.y % o.tiles_n;
     tile<BM, BN, InT, S, T, FAST>(o.v, tm * BM, tn * BN, smem);
@@ -89,7 +96,7 @@ constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
 // variant: 0 = BF16 operands (S=T=1); 1,2,3 = FP32 operands split into 1,2,3 slices (T=S)
 typedef void (*single_fn)(op_view, int64_t, int64_t, int64_t, int, int, uint32_t *, uint32_t, uint32_t);
-typedef void (*grouped_fn)(const gop *, const int2 *, int, unsigned *);
+typedef void (*grouped_fn)(const gop *, const int2 *, const int *, int, unsigned *);
 
 template <int BM, int BN, bool FAST>
 struct cfg_kernels {
@@ -281,9 +288,9 @@ struct bbts_batch {
   struct group {
     int cfg = 0, variant = 0;
     bool fast = true;
-    int nitems = 0, grid = 0;
+    int nitems = 0, grid = 0, nq = 1;
     double flops = 0.0;
-    size_t ops_off = 0, items_off = 0, counter_off = 0;
+    size_t ops_off = 0, items_off = 0, qstart_off = 0, counter_off = 0;
   };
   std::vector<group> groups;
   void *dev = nullptr;
@@ -292,9 +299,6 @@ struct bbts_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<int> order;  // groups by descending work
-  std::vector<hipStream_t> side;
-  std::vector<hipEvent_t> join;
-  hipEvent_t fork = nullptr;
 };
 
 namespace bbts_hip {
@@ -324,6 +328,46 @@ op_view view_of(const bbts_gemm_desc_t &d) {
   v.vecA = al16(d.A) && d.lda % epv == 0;
   v.vecB = al16(d.B) && d.ldb % epv == 0;
   return v;
+}
+
+// Pick a tile config per op from per-(op, config) grouped costs (-1 = not applicable).
+// Plan level first: the single config with the least total cost over the stream; an op
+// moves to its own cheapest config only when that saves at least a quarter of its cost, so
+// the stream does not split into small launches whose tails cost more than they save.
+void choose_configs(bbts_batch *b, const std::vector<float> &cost) {
+  const size_t n = b->ops.size();
+  const float kMargin = 0.75f;
+  std::vector<double> total((size_t)kNumCfgs, 0.0);
+  std::vector<int> usable((size_t)kNumCfgs, 1);
+  for (size_t i = 0; i < n; ++i) {
+    int variant = 0;
+    variant_of(b->ops[i].dtype, b->prec[i], &variant);
+    for (int c = 0; c < kNumCfgs; ++c) {
+      float v = cost[i * kNumCfgs + (size_t)c];
+      if (v < 0 || !valid_cfg(c, variant)) usable[(size_t)c] = 0;
+      else total[(size_t)c] += v;
+    }
+  }
+  int base = -1;
+  for (int c = 0; c < kNumCfgs; ++c)
+    if (usable[(size_t)c] && (base < 0 || total[(size_t)c] < total[(size_t)base])) base = c;
+  for (size_t i = 0; i < n; ++i) {
+    int variant = 0;
+    variant_of(b->ops[i].dtype, b->prec[i], &variant);
+    int best = -1;
+    for (int c = 0; c < kNumCfgs; ++c) {
+      float v = cost[i * kNumCfgs + (size_t)c];
+      if (v < 0 || !valid_cfg(c, variant)) continue;
+      if (best < 0 || v < cost[i * kNumCfgs + (size_t)best]) best = c;
+    }
+    int pick = base;
+    if (pick < 0 || cost[i * kNumCfgs + (size_t)pick] < 0 ||
+        (best >= 0 && cost[i * kNumCfgs + (size_t)best] < kMargin * cost[i * kNumCfgs + (size_t)pick]))
+      pick = best >= 0 ? best : 1;
+    b->cfg[i] = pick;
+    float c = cost[i * kNumCfgs + (size_t)pick];
+    b->tile_us[i] = c > 0 ? c / (float)tiles_of(b->ops[i], pick) : 0.0f;
+  }
 }
 
 }  // namespace
@@ -389,9 +433,6 @@ int bbts_hip_batch_destroy(bbts_batch *b) {
   (void)hipDeviceSynchronize();
   drop_graph(b);
   if (b->dev) (void)hipFree(b->dev);
-  for (auto s : b->side) (void)hipStreamDestroy(s);
-  for (auto e : b->join) (void)hipEventDestroy(e);
-  if (b->fork) (void)hipEventDestroy(b->fork);
   delete b;
   return BBTS_OK;
 }
@@ -443,20 +484,21 @@ int bbts_hip_batch_plan(bbts_batch *b, int policy, bbts_costmodel *cm) {
       b->tile_us[i] = (float)kCfgs[b->cfg[i]].bm * kCfgs[b->cfg[i]].bn * (float)d.K * 1e-9f;
     }
   } else if (policy == BBTS_SCHED_TUNED) {
+    // measured grouped cost of every (op, config) from the tuning table; ops never tuned get
+    // the K * tile-area proxy
+    std::vector<float> cost(n * kNumCfgs, -1.0f);
     for (size_t i = 0; i < n; ++i) {
       const auto &d = b->ops[i];
-      int cfg = -1;
-      float us = 0;
-      bbts_hip_tune_lookup(b->ctx, d.M, d.N, d.K, d.dtype, b->prec[i], &cfg, &us);
-      if (cfg < 0) {
-        cfg = heuristic_cfg(d.M, d.N, d.K);
-        us = (float)kCfgs[cfg].bm * kCfgs[cfg].bn * (float)d.K * 1e-6f;
-      } else {
-        us = us / (float)tiles_of(d, cfg);
+      std::lock_guard<std::mutex> g(b->ctx->mu);
+      auto it = b->ctx->tune_table.find(tune_key{d.M, d.N, d.K, d.dtype, d.dtype == 1 ? 0 : b->prec[i]});
+      for (int c = 0; c < kNumCfgs; ++c) {
+        float v = -1.0f;
+        if (it != b->ctx->tune_table.end() && it->second.all.size() == (size_t)kNumCfgs) v = it->second.all[(size_t)c];
+        if (it == b->ctx->tune_table.end()) v = (float)tiles_of(d, c) * kCfgs[c].bm * kCfgs[c].bn * (float)d.K * 1e-6f;
+        cost[i * kNumCfgs + (size_t)c] = v;
       }
-      b->cfg[i] = cfg;
-      b->tile_us[i] = us;
     }
+    choose_configs(b, cost);
   } else if (policy == BBTS_SCHED_LEARNED) {
     BBTS_REQUIRE(cm != nullptr, BBTS_E_INVALID, "learned policy needs a cost model");
     const int nf = bbts_hip_costmodel_num_features();
@@ -469,24 +511,9 @@ int bbts_hip_batch_plan(bbts_batch *b, int policy, bbts_costmodel *cm) {
     }
     std::vector<float> pred;
     if ((rc = costmodel_predict_host(cm, feats, (int)(n * kNumCfgs), pred))) return rc;
-    for (size_t i = 0; i < n; ++i) {
-      int best = 0;
-      float best_us = 0;
-      int variant = 0;
-      variant_of(b->ops[i].dtype, b->prec[i], &variant);
-      bool first = true;
-      for (int c = 0; c < kNumCfgs; ++c) {
-        if (!valid_cfg(c, variant)) continue;
-        float us = expf(pred[i * kNumCfgs + (size_t)c]);  // predicted grouped cost of the op, microseconds
-        if (first || us < best_us) {
-          first = false;
-          best = c;
-          best_us = us;
-        }
-      }
-      b->cfg[i] = best;
-      b->tile_us[i] = best_us / (float)tiles_of(b->ops[i], best);
-    }
+    std::vector<float> cost(n * kNumCfgs);
+    for (size_t i = 0; i < cost.size(); ++i) cost[i] = expf(pred[i]);  // predicted grouped cost, us
+    choose_configs(b, cost);
   } else {
     set_error("unknown schedule policy %d", policy);
     return BBTS_E_INVALID;
@@ -508,6 +535,7 @@ int bbts_hip_batch_plan(bbts_batch *b, int policy, bbts_costmodel *cm) {
   size_t bytes = 0;
   std::vector<std::vector<gop>> gops;
   std::vector<std::vector<int2>> gitems;
+  std::vector<std::vector<int>> gqstart;
   for (auto &kv : gmap) {
     auto &idx = kv.second;
     if (policy != BBTS_SCHED_FIFO && policy < BBTS_SCHED_FORCED_CFG0) {
@@ -521,22 +549,69 @@ int bbts_hip_batch_plan(bbts_batch *b, int policy, bbts_costmodel *cm) {
     grp.variant = std::get<1>(kv.first);
     grp.fast = std::get<2>(kv.first) != 0;
     std::vector<gop> ops;
-    std::vector<int2> items;
+    std::vector<double> tile_cost;
+    long long total_tiles = 0;
     for (size_t q = 0; q < idx.size(); ++q) {
       const auto &d = b->ops[(size_t)idx[q]];
       gop o{};
       o.v = view_of(d);
       o.tiles_n = (d.N + kCfgs[grp.cfg].bn - 1) / kCfgs[grp.cfg].bn;
-      long long nt = tiles_of(d, grp.cfg);
-      for (long long t = 0; t < nt; ++t) items.push_back(make_int2((int)q, (int)t));
       ops.push_back(o);
+      const double tc = b->tile_us[(size_t)idx[q]] > 0.0f
+                            ? (double)b->tile_us[(size_t)idx[q]]
+                            : 1e-9 * kCfgs[grp.cfg].bm * kCfgs[grp.cfg].bn * (double)(d.K > 0 ? d.K : 1);
+      tile_cost.push_back(tc);
+      total_tiles += tiles_of(d, grp.cfg);
       grp.flops += 2.0 * d.M * d.N * (double)d.K;
     }
+    // Per-XCD queues: workgroup w runs on XCD w % 8 (dispatch is round-robin over the XCDs),
+    // so each op's tiles are kept in contiguous chunks on one queue and the tiles that share A
+    // rows / B columns meet in one XCD's L2.  Chunks are bin-packed largest first onto the
+    // least-loaded queue; inside a queue the order stays largest tile cost first.
+    const int occ = occupancy(grp.cfg, grp.variant);
+    const long long cap = (long long)b->ctx->num_cus * occ;
+    const int nq = (total_tiles >= 64 && cap >= 8 && b->ctx->num_xcds > 1) ? b->ctx->num_xcds : 1;
+    const long long chunk_max = std::max<long long>(1, (total_tiles + 4LL * nq - 1) / (4LL * nq));
+    struct chunk {
+      int op;
+      int t0, t1;
+      double cost;
+    };
+    std::vector<chunk> chunks;
+    for (size_t q = 0; q < ops.size(); ++q) {
+      const auto &d = b->ops[(size_t)idx[q]];
+      const long long nt = tiles_of(d, grp.cfg);
+      for (long long t0 = 0; t0 < nt; t0 += chunk_max) {
+        long long t1 = std::min(nt, t0 + chunk_max);
+        chunks.push_back(chunk{(int)q, (int)t0, (int)t1, tile_cost[q] * (double)(t1 - t0)});
+      }
+    }
+    std::stable_sort(chunks.begin(), chunks.end(), [](const chunk &x, const chunk &y) { return x.cost > y.cost; });
+    std::vector<std::vector<const chunk *>> queue((size_t)nq);
+    std::vector<double> load((size_t)nq, 0.0);
+    for (const chunk &c : chunks) {
+      size_t best = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+      queue[best].push_back(&c);
+      load[best] += c.cost;
+    }
+    std::vector<int2> items;
+    std::vector<int> qstart((size_t)nq + 1, 0);
+    for (int x = 0; x < nq; ++x) {
+      auto &qq = queue[(size_t)x];
+      if (policy != BBTS_SCHED_FIFO && policy < BBTS_SCHED_FORCED_CFG0)
+        std::stable_sort(qq.begin(), qq.end(), [&](const chunk *p, const chunk *r) {
+          return tile_cost[(size_t)p->op] > tile_cost[(size_t)r->op];
+        });
+      qstart[(size_t)x] = (int)items.size();
+      for (const chunk *c : qq)
+        for (int t = c->t0; t < c->t1; ++t) items.push_back(make_int2(c->op, t));
+    }
+    qstart[(size_t)nq] = (int)items.size();
     grp.nitems = (int)items.size();
-    int occ = occupancy(grp.cfg, grp.variant);
-    long long cap = (long long)b->ctx->num_cus * occ;
-    grp.grid = (int)(grp.nitems < cap ? grp.nitems : cap);
-    if (grp.grid < 1) grp.grid = 1;
+    grp.nq = nq;
+    long long grid = grp.nitems < cap ? grp.nitems : cap;
+    grid -= grid % nq;
+    grp.grid = (int)std::max<long long>(grid, nq);
     bytes = (bytes + 255) / 256 * 256;
     grp.ops_off = bytes;
     bytes += ops.size() * sizeof(gop);
@@ -544,8 +619,12 @@ int bbts_hip_batch_plan(bbts_batch *b, int policy, bbts_costmodel *cm) {
     grp.items_off = bytes;
     bytes += items.size() * sizeof(int2);
     bytes = (bytes + 255) / 256 * 256;
+    grp.qstart_off = bytes;
+    bytes += qstart.size() * sizeof(int);
+    bytes = (bytes + 255) / 256 * 256;
     grp.counter_off = bytes;
-    bytes += 256;
+    bytes += 64 * sizeof(unsigned) * (size_t)nq;  // one 256-byte line per queue counter
+    gqstart.push_back(std::move(qstart));
     b->groups.push_back(grp);
     gops.push_back(std::move(ops));
     gitems.push_back(std::move(items));
@@ -565,6 +644,7 @@ int bbts_hip_batch_plan(bbts_batch *b, int policy, bbts_costmodel *cm) {
       auto &grp = b->groups[gi];
       std::memcpy(host.data() + grp.ops_off, gops[gi].data(), gops[gi].size() * sizeof(gop));
       std::memcpy(host.data() + grp.items_off, gitems[gi].data(), gitems[gi].size() * sizeof(int2));
+      std::memcpy(host.data() + grp.qstart_off, gqstart[gi].data(), gqstart[gi].size() * sizeof(int));
     }
     BBTS_HIP_CHECK(hipMemcpy(b->dev, host.data(), bytes, hipMemcpyHostToDevice));
   }
@@ -587,7 +667,8 @@ int bbts_hip_batch_launch_group(bbts_batch *b, int g, bbts_stream_t stream) {
   if ((rc = ensure_smem((const void *)fn, bytes))) return rc;
   hipLaunchKernelGGL(fn, dim3((unsigned)grp.grid), dim3(256), (unsigned)bytes, (hipStream_t)stream,
                      reinterpret_cast<const gop *>(base + grp.ops_off),
-                     reinterpret_cast<const int2 *>(base + grp.items_off), grp.nitems,
+                     reinterpret_cast<const int2 *>(base + grp.items_off),
+                     reinterpret_cast<const int *>(base + grp.qstart_off), grp.nq,
                      reinterpret_cast<unsigned *>(base + grp.counter_off));
   BBTS_LAUNCH_CHECK();
   return BBTS_OK;
@@ -610,8 +691,8 @@ int bbts_hip_batch_launch(bbts_batch *b, bbts_stream_t stream) {
   int rc = bind(b->ctx);
   if (rc) return rc;
   if (!b->planned && (rc = bbts_hip_batch_plan(b, BBTS_SCHED_HEURISTIC, nullptr))) return rc;
-  for (int g = 0; g < (int)b->groups.size(); ++g)
-    if ((rc = bbts_hip_batch_launch_group(b, g, stream))) return rc;
+  for (size_t g = 0; g < b->order.size(); ++g)
+    if ((rc = bbts_hip_batch_launch_group(b, b->order[g], stream))) return rc;
   return BBTS_OK;
 }
 
@@ -633,39 +714,11 @@ int bbts_hip_batch_capture(bbts_batch *b, bbts_stream_t stream) {
     BBTS_HIP_CHECK(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
     cap = own;
   }
-  // one side stream + join event per extra group: the graph launches the groups
-  // concurrently (fork from the capture stream, join back), so the persistent grid of one
-  // tile configuration fills the CUs another group's tail leaves idle
-  const int ng = (int)b->groups.size();
-  while ((int)b->side.size() < ng - 1) {
-    hipStream_t ss;
-    hipEvent_t ev;
-    BBTS_HIP_CHECK(hipStreamCreateWithFlags(&ss, hipStreamNonBlocking));
-    BBTS_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    b->side.push_back(ss);
-    b->join.push_back(ev);
-  }
-  if (!b->fork) BBTS_HIP_CHECK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+  // The groups run one after the other, largest first.  (Launching them concurrently on
+  // forked streams was measured slower: a co-running grid breaks the workgroup-to-XCD
+  // placement the per-XCD queues rely on.)
   BBTS_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-  if (ng > 1) {
-    // largest group on the capture stream, the others on side streams
-    hipError_t e0 = hipEventRecord(b->fork, cap);
-    for (int g = 1; g < ng && rc == BBTS_OK && e0 == hipSuccess; ++g) {
-      hipStream_t ss = b->side[(size_t)(g - 1)];
-      e0 = hipStreamWaitEvent(ss, b->fork, 0);
-      if (e0 == hipSuccess) rc = bbts_hip_batch_launch_group(b, b->order[(size_t)g], ss);
-      if (e0 == hipSuccess && rc == BBTS_OK) e0 = hipEventRecord(b->join[(size_t)(g - 1)], ss);
-    }
-    if (rc == BBTS_OK && e0 == hipSuccess) rc = bbts_hip_batch_launch_group(b, b->order[0], cap);
-    for (int g = 1; g < ng && rc == BBTS_OK && e0 == hipSuccess; ++g)
-      e0 = hipStreamWaitEvent(cap, b->join[(size_t)(g - 1)], 0);
-    if (e0 != hipSuccess && rc == BBTS_OK) {
-      set_error("graph fork/join failed: %s", hipGetErrorString(e0));
-      rc = BBTS_E_HIP;
-    }
-  } else {
-    rc = bbts_hip_batch_launch(b, cap);
-  }
+  rc = bbts_hip_batch_launch(b, cap);
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(cap, &graph);
   if (own) (void)hipStreamDestroy(own);
@@ -739,6 +792,7 @@ int bbts_hip_tune_batch(bbts_batch *b, int reps, float *times_us) {
     if ((rc = variant_of(d.dtype, prec, &variant))) break;
     int best = -1;
     float best_us = 0.0f;
+    std::vector<float> all((size_t)kNumCfgs, -1.0f);
     const bool fast = fast_ok(view_of(d), d.dtype == 1 ? 8 : 4);
     for (int c = 0; c < kNumCfgs; ++c) {
       // configs that do not fit LDS, or that the element-wise path maps elsewhere, are not timed
@@ -766,6 +820,7 @@ int bbts_hip_tune_batch(bbts_batch *b, int reps, float *times_us) {
       float ms = 0.0f;
       hipEventElapsedTime(&ms, e0, e1);
       float us = ms * 1000.0f / (float)(reps * R);
+      all[(size_t)c] = us;
       if (times_us)
         for (size_t i : kv.second) times_us[i * kNumCfgs + (size_t)c] = us;
       if (best < 0 || us < best_us) {
@@ -775,7 +830,7 @@ int bbts_hip_tune_batch(bbts_batch *b, int reps, float *times_us) {
     }
     if (rc) break;
     std::lock_guard<std::mutex> g(b->ctx->mu);
-    b->ctx->tune_table[kv.first] = tune_val{best, best_us};
+    b->ctx->tune_table[kv.first] = tune_val{best, best_us, all};
   }
   (void)hipStreamSynchronize(s);
   if (tmp.dev) (void)hipFree(tmp.dev);

@@ -199,6 +199,25 @@ def cpu_baseline(shapes, budget_s=15.0):
                       % (n_ops, len(shapes), done_flops / 1e9, O.sgemm_provider(), spent)}
 
 
+def pmc_run(torch, ctx, shapes, seed, cm):
+    """Counter pass: the learned plan launched twice, then one plain copy of 256 MiB (the
+    FETCH_SIZE / WRITE_SIZE calibration: its algorithmic bytes are 256 MiB read + 256 MiB written)."""
+    from bbts_amd import call
+    st = Stream(torch, ctx, shapes, seed, fp32_copies=False)
+    b = make_batch(ctx, st)
+    b.plan(3, cm)
+    b.launch()
+    b.launch()
+    torch.cuda.synchronize()
+    src = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    arr = (ctypes.c_void_p * 1)(dst.data_ptr())
+    call("bbts_hip_copy_n", ctx.h, src.data_ptr(), src.numel() * 4, arr, 1, None)
+    torch.cuda.synchronize()
+    b.close()
+    print("pmc pass done", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,6 +228,11 @@ def main():
     ap.add_argument("--train-ops", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ablation", action="store_true")
+    ap.add_argument("--save-costmodel", default=None, help="write the trained cost-model parameters (.npy)")
+    ap.add_argument("--load-costmodel", default=None,
+                    help="use these cost-model parameters instead of tuning + training (profiling runs)")
+    ap.add_argument("--pmc", action="store_true",
+                    help="counter run: one learned pass plus a 256 MiB copy kernel for calibration, no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU's share of the block stream; multi-GPU runs are not part of this "
@@ -226,8 +250,19 @@ def main():
     total_flops = flops_of(shapes)
     print("stream: %d ops, %.3f TFLOP" % (len(shapes), total_flops / 1e12), file=sys.stderr, flush=True)
 
-    cm = train_cost_model(torch, ctx, args.train_ops, args.seed + 1000, log)
-    print("cost model trained: %s" % log["costmodel"], file=sys.stderr, flush=True)
+    if args.load_costmodel:
+        from bbts_amd.device import CostModel
+        cm = CostModel(ctx, hidden=64, seed=0)
+        cm.set_params(np.load(args.load_costmodel, allow_pickle=False))
+        log["costmodel"] = {"loaded": os.path.basename(args.load_costmodel)}
+    else:
+        cm = train_cost_model(torch, ctx, args.train_ops, args.seed + 1000, log)
+        if args.save_costmodel:
+            np.save(args.save_costmodel, cm.get_params(), allow_pickle=False)
+    print("cost model: %s" % log["costmodel"], file=sys.stderr, flush=True)
+    if args.pmc:
+        pmc_run(torch, ctx, shapes, args.seed, cm)
+        return
 
     st = Stream(torch, ctx, shapes, args.seed)
     s = torch.cuda.current_stream()
