@@ -12,6 +12,7 @@
 // node and no host work.
 #include "gemm_bf16.cuh"
 
+#include <cmath>
 #include <tuple>
 
 namespace bbts_hip {
@@ -208,6 +209,27 @@ int heuristic_cfg(int M, int N, int K) {
   return 3;
 }
 
+// Tile for one stand-alone launch: the config with the least estimated time, counting whole
+// waves of tiles over the chip (tiles / (CUs x occupancy), rounded up) times the tile's work
+// divided by its measured relative efficiency (4096^3 single-config runs: 128x128 best).
+int single_op_cfg(int M, int N, int K, int batch, int variant, int num_cus) {
+  static const double kEff[] = {0.79, 1.0, 0.80, 0.74, 0.80, 0.63};
+  int best = 1;
+  double best_t = -1.0;
+  for (int c = 0; c < kNumCfgs; ++c) {
+    if (!valid_cfg(c, variant)) continue;
+    const double tiles = (double)((M + kCfgs[c].bm - 1) / kCfgs[c].bm) * ((N + kCfgs[c].bn - 1) / kCfgs[c].bn) * batch;
+    const double slots = (double)num_cus * occupancy(c, variant);
+    const double waves = std::ceil(tiles / slots);
+    const double t = waves * kCfgs[c].bm * kCfgs[c].bn * (double)(K > 0 ? K : 1) / kEff[c];
+    if (best_t < 0 || t < best_t) {
+      best = c;
+      best_t = t;
+    }
+  }
+  return best;
+}
+
 }  // namespace
 
 int launch_gemm_split_bf16(bbts_hip_ctx *ctx, const gemm_args &g, int slices_a, int slices_b, int terms,
@@ -246,7 +268,7 @@ int launch_gemm_split_bf16(bbts_hip_ctx *ctx, const gemm_args &g, int slices_a, 
       ldb = g.N;
     }
   }
-  int cfg = heuristic_cfg(g.M, g.N, g.K);
+  int cfg = single_op_cfg(g.M, g.N, g.K, g.batch, variant, ctx->num_cus);
   op_view v{};
   v.A = A; v.B = B; v.C = g.C;
   v.lda = lda; v.ldb = ldb; v.ldc = g.ldc;

@@ -20,8 +20,7 @@ namespace {
 
 typedef float f16v __attribute__((ext_vector_type(16)));
 
-constexpr int BM = 128, BN = 128, BK = 16;
-constexpr int LDS_STRIDE = 160;
+constexpr int BK = 16;
 constexpr int kThreads = 256;
 constexpr int kGroupM = 8;
 
@@ -40,13 +39,21 @@ __device__ inline void map_tile(int lin, int tiles_m, int tiles_n, int *tm, int 
   *tn = in_group / gsize;
 }
 
+// TB x TB output tile (TB = 128: 64x64 per wave; TB = 64: 32x32 per wave, for blocks too
+// small to fill 256 CUs with 128x128 tiles).  LDS rows are TB + 32 floats, which keeps the
+// two half-waves of an MFMA operand read in disjoint banks.
 // AT: A stored K x M (element (m,k) at A[k*lda + m]); else A[m*lda + k].
 // BT: B stored N x K (element (k,n) at B[n*ldb + k]); else B[k*ldb + n].
-template <bool AT, bool BT>
+template <int TB, bool AT, bool BT>
 __global__ __launch_bounds__(kThreads, 2) void sgemm_kernel(gemm_args g, int vecA, int vecB, int tiles_m,
                                                             int tiles_n) {
-  __shared__ float As[2][BK][LDS_STRIDE];
-  __shared__ float Bs[2][BK][LDS_STRIDE];
+  constexpr int STRIDE = TB + 32;
+  constexpr int W = TB / 2;        // per-wave tile
+  constexpr int F = W / 32;        // 32x32 MFMA tiles per wave side
+  constexpr int PER = TB * BK / 4 / kThreads;  // float4 per lane per operand and stage
+  constexpr int RV = TB / 4;       // float4 per MN-contiguous row
+  __shared__ float As[2][BK][STRIDE];
+  __shared__ float Bs[2][BK][STRIDE];
 
   if (g.meta_dst != nullptr && blockIdx.x == 0 && blockIdx.z == 0 && threadIdx.x == 0) {
     g.meta_dst[0] = g.w0;
@@ -60,17 +67,17 @@ __global__ __launch_bounds__(kThreads, 2) void sgemm_kernel(gemm_args g, int vec
   const float *__restrict__ B = g.B + (int64_t)bz * g.sb;
   float *__restrict__ C = g.C + (int64_t)bz * g.sc;
   const int M = g.M, N = g.N, K = g.K;
-  const int m0 = tm * BM, n0 = tn * BN;
+  const int m0 = tm * TB, n0 = tn * TB;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
 
-  float ra[8], rb[8];
+  float ra[4 * PER], rb[4 * PER];
 
   auto load_a = [&](int k0) {
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
+    for (int r = 0; r < PER; ++r) {
       int q = tid + kThreads * r;
       if (!AT) {
         int m = q >> 2, kq = (q & 3) * 4;
@@ -84,7 +91,7 @@ __global__ __launch_bounds__(kThreads, 2) void sgemm_kernel(gemm_args g, int vec
           for (int c = 0; c < 4; ++c) ra[4 * r + c] = (gm < M && gk + c < K) ? p[c] : 0.0f;
         }
       } else {
-        int k = q >> 5, mq = (q & 31) * 4;
+        int k = q / RV, mq = (q % RV) * 4;
         int gm = m0 + mq, gk = k0 + k;
         const float *p = A + (int64_t)gk * g.lda + gm;
         if (vecA && gk < K && gm + 3 < M) {
@@ -99,10 +106,10 @@ __global__ __launch_bounds__(kThreads, 2) void sgemm_kernel(gemm_args g, int vec
   };
   auto load_b = [&](int k0) {
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
+    for (int r = 0; r < PER; ++r) {
       int q = tid + kThreads * r;
       if (!BT) {
-        int k = q >> 5, nq = (q & 31) * 4;
+        int k = q / RV, nq = (q % RV) * 4;
         int gk = k0 + k, gn = n0 + nq;
         const float *p = B + (int64_t)gk * g.ldb + gn;
         if (vecB && gk < K && gn + 3 < N) {
@@ -128,19 +135,19 @@ __global__ __launch_bounds__(kThreads, 2) void sgemm_kernel(gemm_args g, int vec
   };
   auto store_lds = [&](int buf) {
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
+    for (int r = 0; r < PER; ++r) {
       int q = tid + kThreads * r;
       if (!AT) {
         int m = q >> 2, kq = (q & 3) * 4;
 #pragma unroll
         for (int c = 0; c < 4; ++c) As[buf][kq + c][m] = ra[4 * r + c];
       } else {
-        int k = q >> 5, mq = (q & 31) * 4;
+        int k = q / RV, mq = (q % RV) * 4;
         *reinterpret_cast<float4 *>(&As[buf][k][mq]) =
             float4{ra[4 * r + 0], ra[4 * r + 1], ra[4 * r + 2], ra[4 * r + 3]};
       }
       if (!BT) {
-        int k = q >> 5, nq = (q & 31) * 4;
+        int k = q / RV, nq = (q % RV) * 4;
         *reinterpret_cast<float4 *>(&Bs[buf][k][nq]) =
             float4{rb[4 * r + 0], rb[4 * r + 1], rb[4 * r + 2], rb[4 * r + 3]};
       } else {
@@ -151,11 +158,11 @@ __global__ __launch_bounds__(kThreads, 2) void sgemm_kernel(gemm_args g, int vec
     }
   };
 
-  f16v acc[2][2];
+  f16v acc[F][F];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < F; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < F; ++j)
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
 
@@ -177,14 +184,15 @@ __global__ __launch_bounds__(kThreads, 2) void sgemm_kernel(gemm_args g, int vec
 #pragma unroll
     for (int kk = 0; kk < BK / 2; ++kk) {
       const int k = kk * 2 + half;
-      float a0 = As[buf][k][wm * 64 + col];
-      float a1 = As[buf][k][wm * 64 + 32 + col];
-      float b0 = Bs[buf][k][wn * 64 + col];
-      float b1 = Bs[buf][k][wn * 64 + 32 + col];
-      acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[1][1], 0, 0, 0);
+      float a[F], b[F];
+#pragma unroll
+      for (int i = 0; i < F; ++i) a[i] = As[buf][k][wm * W + i * 32 + col];
+#pragma unroll
+      for (int j = 0; j < F; ++j) b[j] = Bs[buf][k][wn * W + j * 32 + col];
+#pragma unroll
+      for (int i = 0; i < F; ++i)
+#pragma unroll
+        for (int j = 0; j < F; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], b[j], acc[i][j], 0, 0, 0);
     }
     if (kt + 1 < nk) store_lds(buf ^ 1);
     __syncthreads();
@@ -193,14 +201,14 @@ __global__ __launch_bounds__(kThreads, 2) void sgemm_kernel(gemm_args g, int vec
   // Epilogue: accumulator v of a 32x32 tile holds row 8*(v/4) + 4*(lane/32) + v%4, column lane%32.
   const float alpha = g.alpha;
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < F; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int gn = n0 + wn * 64 + j * 32 + col;
+    for (int j = 0; j < F; ++j) {
+      const int gn = n0 + wn * W + j * 32 + col;
       if (gn >= N) continue;
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
-        const int gm = m0 + wm * 64 + i * 32 + (v >> 2) * 8 + half * 4 + (v & 3);
+        const int gm = m0 + wm * W + i * 32 + (v >> 2) * 8 + half * 4 + (v & 3);
         if (gm < M) C[(int64_t)gm * g.ldc + gn] = alpha * acc[i][j][v];
       }
     }
@@ -208,25 +216,36 @@ __global__ __launch_bounds__(kThreads, 2) void sgemm_kernel(gemm_args g, int vec
 
 inline bool al16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
+template <int TB>
+void launch_tiles(const gemm_args &g, dim3 grid, hipStream_t s, int vecA, int vecB, int tiles_m, int tiles_n) {
+  if (!g.ta && !g.tb)
+    hipLaunchKernelGGL((sgemm_kernel<TB, false, false>), grid, dim3(kThreads), 0, s, g, vecA, vecB, tiles_m, tiles_n);
+  else if (g.ta && !g.tb)
+    hipLaunchKernelGGL((sgemm_kernel<TB, true, false>), grid, dim3(kThreads), 0, s, g, vecA, vecB, tiles_m, tiles_n);
+  else if (!g.ta && g.tb)
+    hipLaunchKernelGGL((sgemm_kernel<TB, false, true>), grid, dim3(kThreads), 0, s, g, vecA, vecB, tiles_m, tiles_n);
+  else
+    hipLaunchKernelGGL((sgemm_kernel<TB, true, true>), grid, dim3(kThreads), 0, s, g, vecA, vecB, tiles_m, tiles_n);
+}
+
 }  // namespace
 
 int launch_sgemm_f32(bbts_hip_ctx *ctx, const gemm_args &g, hipStream_t s) {
-  (void)ctx;
   BBTS_REQUIRE(g.M >= 0 && g.N >= 0 && g.K >= 0 && g.batch >= 1, BBTS_E_INVALID, "bad GEMM sizes");
   BBTS_REQUIRE(g.batch <= 65535, BBTS_E_INVALID, "batch %d exceeds 65535", g.batch);
   if (g.M == 0 || g.N == 0) return BBTS_OK;
-  int tiles_m = (g.M + BM - 1) / BM, tiles_n = (g.N + BN - 1) / BN;
+  // 64x64 tiles when 128x128 tiles would leave CUs idle (fewer tiles than CUs)
+  const long long t128 = (long long)((g.M + 127) / 128) * ((g.N + 127) / 128) * g.batch;
+  const int cus = ctx ? ctx->num_cus : 256;
+  const int TB = t128 < (long long)cus ? 64 : 128;
+  int tiles_m = (g.M + TB - 1) / TB, tiles_n = (g.N + TB - 1) / TB;
   int vecA = al16(g.A) && (g.lda % 4 == 0) && (g.batch == 1 || g.sa % 4 == 0);
   int vecB = al16(g.B) && (g.ldb % 4 == 0) && (g.batch == 1 || g.sb % 4 == 0);
   dim3 grid((unsigned)(tiles_m * tiles_n), 1, (unsigned)g.batch);
-  if (!g.ta && !g.tb)
-    hipLaunchKernelGGL((sgemm_kernel<false, false>), grid, dim3(kThreads), 0, s, g, vecA, vecB, tiles_m, tiles_n);
-  else if (g.ta && !g.tb)
-    hipLaunchKernelGGL((sgemm_kernel<true, false>), grid, dim3(kThreads), 0, s, g, vecA, vecB, tiles_m, tiles_n);
-  else if (!g.ta && g.tb)
-    hipLaunchKernelGGL((sgemm_kernel<false, true>), grid, dim3(kThreads), 0, s, g, vecA, vecB, tiles_m, tiles_n);
+  if (TB == 64)
+    launch_tiles<64>(g, grid, s, vecA, vecB, tiles_m, tiles_n);
   else
-    hipLaunchKernelGGL((sgemm_kernel<true, true>), grid, dim3(kThreads), 0, s, g, vecA, vecB, tiles_m, tiles_n);
+    launch_tiles<128>(g, grid, s, vecA, vecB, tiles_m, tiles_n);
   BBTS_LAUNCH_CHECK();
   return BBTS_OK;
 }

@@ -199,6 +199,96 @@ def cpu_baseline(shapes, budget_s=15.0):
                       % (n_ops, len(shapes), done_flops / 1e9, O.sgemm_provider(), spent)}
 
 
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def suite(torch, ctx, steps):
+    """One JSON line per hot-path kernel at its block size, against its roofline."""
+    from bbts_amd import call
+    s = torch.cuda.current_stream()
+    sh = ctypes.c_void_p(s.cuda_stream)
+    out = []
+
+    def tm(fn, reps=steps):
+        return time_passes(torch, fn, reps, 2, s)
+
+    def emit(name, work, unit, ms, peak, extra=None):
+        rate = work / ms / 1e6 if unit == "GB/s" else work / ms / 1e9
+        d = {"kernel": name, "unit": unit, "achieved": round(rate, 2), "peak": peak,
+             "frac": round(rate / peak, 4), "ms": round(ms, 4)}
+        if extra:
+            d.update(extra)
+        out.append(d)
+        print(json.dumps(d), flush=True)
+
+    n = 2048
+    mn = n * n
+    blocks = [ctx.dense(n, n) for _ in range(9)]
+    for q, b in enumerate(blocks):
+        call("bbts_hip_gen_normal", ctx.h, b.data_ptr, mn, 100 + q, 0, 0.0, 1.0, None)
+    outb = ctx.dense(n, n)
+    ms = tm(lambda: call("bbts_hip_dense_add", ctx.h, blocks[0].ptr, blocks[1].ptr, outb.ptr, sh))
+    emit("dense_add 2048x2048", 12.0 * mn, "GB/s", ms, HBM_PEAK_GBS)
+    arr = (ctypes.c_void_p * 8)(*[b.ptr for b in blocks[:8]])
+    ms = tm(lambda: call("bbts_hip_dense_reduce", ctx.h, arr, 8, outb.ptr, sh))
+    emit("dense_reduce k=8 2048x2048 (fused left fold)", 9 * 4.0 * mn, "GB/s", ms, HBM_PEAK_GBS,
+         {"pairwise_equivalent_bytes": 3 * 7 * 4 * mn})
+    ms = tm(lambda: call("bbts_hip_dense_relu", ctx.h, blocks[0].ptr, outb.ptr, sh))
+    emit("dense_relu 2048x2048", 8.0 * mn, "GB/s", ms, HBM_PEAK_GBS)
+    dsts = [ctx.dense(n, n) for _ in range(32)]
+    darr = (ctypes.c_void_p * 32)(*[d.ptr for d in dsts])
+    tb = 260 + 4 * mn
+    ms = tm(lambda: call("bbts_hip_broadcast", ctx.h, blocks[0].ptr, darr, 32, sh))
+    emit("broadcast 2048x2048 block -> 32", 33.0 * tb, "GB/s", ms, HBM_PEAK_GBS)
+    for d in dsts:
+        d.free()
+
+    # block GEMMs: FP32 drop-in on the C1 / C2 block shapes, split-BF16 forms on FP32 N(0,1)
+    out2 = torch.zeros(2, dtype=torch.float64, device="cuda")
+    for (sz, precs) in ((1024, (0,)), (2048, (0, 2, 3))):
+        a, bb = (torch.empty(sz * sz, dtype=torch.float32, device="cuda") for _ in range(2))
+        ctx.gen_normal(a, 7)
+        ctx.gen_normal(bb, 8)
+        c = torch.empty(sz * sz, dtype=torch.float32, device="cuda")
+        for prec in precs:
+            ms = tm(lambda: call("bbts_hip_gemm", ctx.h, sz, sz, sz, a.data_ptr(), sz, 0, bb.data_ptr(), sz, 0,
+                                 c.data_ptr(), sz, 1.0, prec, sh))
+            ctx.residual(sz, sz, sz, a, sz, 0, bb, sz, 0, c, sz, out2)
+            torch.cuda.synchronize()
+            v = out2.cpu().numpy()
+            name = {0: "sgemm FP32 MFMA", 2: "gemm BF16x3 split", 3: "gemm BF16x6 split"}[prec]
+            emit("%s %d^3" % (name, sz), 2.0 * sz ** 3, "TFLOP/s", ms, FP32_PEAK_TFLOPS if prec == 0 else 2500.0,
+                 {"rel_err_fp64": math.sqrt(v[0] / v[1]), "note": "FP32-equivalent flops"})
+        del a, bb, c
+
+    # contraction abcd,cdef->abef, a=b=e=f=96, c=d=128, FP32 N(0,1), column-major
+    dl, dr = [96, 96, 128, 128], [128, 128, 96, 96]
+    L = torch.empty(96 * 96 * 128 * 128, dtype=torch.float32, device="cuda")
+    R = torch.empty_like(L)
+    O = torch.empty(96 ** 4, dtype=torch.float32, device="cuda")
+    ctx.gen_normal(L, 11)
+    ctx.gen_normal(R, 12)
+    words = [4, 4, 4, 0, 1, 2, 3, 2, 3, 4, 5, 0, 1, 4, 5, 0x3F800000]
+    p = (ctypes.c_int32 * len(words))(*words)
+    pdl = (ctypes.c_int64 * 4)(*dl)
+    pdr = (ctypes.c_int64 * 4)(*dr)
+    flops = 2.0 * 96 * 96 * 128 * 128 * 96 * 96
+    for prec in (0, 2, 3):
+        ms = tm(lambda: call("bbts_hip_contraction_raw", ctx.h, p, len(words), 4, pdl, L.data_ptr(), 4, pdr,
+                             R.data_ptr(), O.data_ptr(), prec, sh), reps=max(2, steps // 3))
+        # column-major out[ab, ef] is row-major [ef][ab] = rhs^T (row-major [ef][cd]) . lhs^T ([cd][ab])
+        ctx.residual(9216, 9216, 16384, R, 16384, 0, L, 9216, 0, O, 9216, out2)
+        torch.cuda.synchronize()
+        v = out2.cpu().numpy()
+        emit("contraction abcd,cdef->abef (96,96,128,128) %s" % {0: "FP32", 2: "BF16x3", 3: "BF16x6"}[prec],
+             flops, "TFLOP/s", ms, FP32_PEAK_TFLOPS if prec == 0 else 2500.0,
+             {"rel_err_fp64": math.sqrt(v[0] / v[1])})
+    for b in blocks + [outb]:
+        b.free()
+    return out
+
+
 def pmc_run(torch, ctx, shapes, seed, cm):
     """Counter pass: the learned plan launched twice, then one plain copy of 256 MiB (the
     FETCH_SIZE / WRITE_SIZE calibration: its algorithmic bytes are 256 MiB read + 256 MiB written)."""
@@ -231,6 +321,8 @@ def main():
     ap.add_argument("--save-costmodel", default=None, help="write the trained cost-model parameters (.npy)")
     ap.add_argument("--load-costmodel", default=None,
                     help="use these cost-model parameters instead of tuning + training (profiling runs)")
+    ap.add_argument("--suite", action="store_true",
+                    help="per-kernel lines (add, reduce, ReLU, broadcast, GEMM forms, contraction) instead")
     ap.add_argument("--pmc", action="store_true",
                     help="counter run: one learned pass plus a 256 MiB copy kernel for calibration, no JSON")
     args = ap.parse_args()
@@ -245,6 +337,9 @@ def main():
 
     ctx = Context(0)
     info = ctx.info()
+    if args.suite:
+        suite(torch, ctx, args.steps)
+        return
     log = {}
     shapes = stream_shapes(args.ops, args.seed)
     total_flops = flops_of(shapes)
