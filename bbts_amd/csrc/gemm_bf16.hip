@@ -209,25 +209,20 @@ int heuristic_cfg(int M, int N, int K) {
   return 3;
 }
 
-// Tile for one stand-alone launch: the config with the least estimated time, counting whole
-// waves of tiles over the chip (tiles / (CUs x occupancy), rounded up) times the tile's work
-// divided by its measured relative efficiency (4096^3 single-config runs: 128x128 best).
+// Tile for one stand-alone launch, by how many 128x128 output tiles the launch has
+// against the chip's CUs. Thresholds from single-launch sweeps on MI355X (256 CUs) over
+// cubic and 1152x1152x2048 shapes: below two tiles per CU the small 64x64 tile wins (it
+// puts four workgroups on every CU); for the 3-product split a 128x64 tile wins up to four
+// tiles per CU; for the 6-product split, whose 128x128 kernel is register-limited, the
+// 64x128 tile and then the 256x128 tile win. Configs: 0 256x128, 1 128x128, 2 128x64,
+// 3 64x64, 4 64x128.
 int single_op_cfg(int M, int N, int K, int batch, int variant, int num_cus) {
-  static const double kEff[] = {0.79, 1.0, 0.80, 0.74, 0.80, 0.63};
-  int best = 1;
-  double best_t = -1.0;
-  for (int c = 0; c < kNumCfgs; ++c) {
-    if (!valid_cfg(c, variant)) continue;
-    const double tiles = (double)((M + kCfgs[c].bm - 1) / kCfgs[c].bm) * ((N + kCfgs[c].bn - 1) / kCfgs[c].bn) * batch;
-    const double slots = (double)num_cus * occupancy(c, variant);
-    const double waves = std::ceil(tiles / slots);
-    const double t = waves * kCfgs[c].bm * kCfgs[c].bn * (double)(K > 0 ? K : 1) / kEff[c];
-    if (best_t < 0 || t < best_t) {
-      best = c;
-      best_t = t;
-    }
-  }
-  return best;
+  (void)K;
+  const long long t128 = (long long)((M + 127) / 128) * ((N + 127) / 128) * batch;
+  const long long cus = num_cus > 0 ? num_cus : 256;
+  if (variant <= 1) return t128 < 2 * cus ? 3 : 1;
+  if (variant == 2) return t128 < cus ? 3 : (t128 < 4 * cus ? 2 : 1);
+  return t128 < cus ? 3 : (t128 < 4 * cus ? 4 : 0);
 }
 
 }  // namespace
@@ -387,6 +382,24 @@ void choose_configs(bbts_batch *b, const std::vector<float> &cost) {
         (best >= 0 && cost[i * kNumCfgs + (size_t)best] < kMargin * cost[i * kNumCfgs + (size_t)pick]))
       pick = best >= 0 ? best : 1;
     b->cfg[i] = pick;
+  }
+  // Every config other than the base becomes a launch of its own, which adds a launch gap and
+  // a drain tail (about 25 us on a 256-CU part for a small group). Ops leave the base config
+  // only when their summed saving pays for that.
+  const double kGroupUs = 25.0;
+  if (base >= 0) {
+    std::vector<double> saving((size_t)kNumCfgs, 0.0);
+    for (size_t i = 0; i < n; ++i) {
+      const int c = b->cfg[i];
+      if (c != base) saving[(size_t)c] += (double)cost[i * kNumCfgs + (size_t)base] - cost[i * kNumCfgs + (size_t)c];
+    }
+    for (size_t i = 0; i < n; ++i) {
+      const int c = b->cfg[i];
+      if (c != base && saving[(size_t)c] < kGroupUs) b->cfg[i] = base;
+    }
+  }
+  for (size_t i = 0; i < n; ++i) {
+    const int pick = b->cfg[i];
     float c = cost[i * kNumCfgs + (size_t)pick];
     b->tile_us[i] = c > 0 ? c / (float)tiles_of(b->ops[i], pick) : 0.0f;
   }

@@ -21,6 +21,7 @@ import ctypes
 import json
 import math
 import os
+import re
 import sys
 import time
 
@@ -289,6 +290,26 @@ def suite(torch, ctx, steps):
     return out
 
 
+TRAFFIC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "gemm_grouped_bf16_stream_pmc.md")
+
+
+def measured_traffic(ops, seed, cfg):
+    """HBM bytes per launch of the dominant grouped launch, from the committed FETCH_SIZE /
+    WRITE_SIZE counter summary (each counter scaled by what it reads for a plain copy of known
+    size). None when the summary is absent or was taken on another workload or tile config."""
+    try:
+        text = open(TRAFFIC_SUMMARY).read()
+    except OSError:
+        return None
+    kv = dict(re.findall(r"^(\w+): (.+)$", text, re.M))
+    if kv.get("workload") != "ops=%d seed=%d cfg=%d" % (ops, seed, cfg):
+        return None
+    kib = (float(kv["fetch_kib_per_dispatch"]) * float(kv["fetch_factor"]) +
+           float(kv["write_kib_per_dispatch"]) * float(kv["write_factor"]))
+    return int(kib * 1024)
+
+
 def pmc_run(torch, ctx, shapes, seed, cm):
     """Counter pass: the learned plan launched twice, then one plain copy of 256 MiB (the
     FETCH_SIZE / WRITE_SIZE calibration: its algorithmic bytes are 256 MiB read + 256 MiB written)."""
@@ -406,6 +427,11 @@ def main():
                        "ms": ms_g})
     dom = max(groups, key=lambda x: x["ms"])
     achieved = dom["gflop"] / dom["ms"]  # GFLOP per ms == TFLOP/s
+    # algorithmic bytes of the dominant launch: each operand read once (BF16), each output
+    # written once (FP32), over the ops planned onto that launch's tile config
+    dom_bytes = sum(2 * M * K + 2 * K * N + 4 * M * N for i, (M, N, K) in enumerate(shapes)
+                    if b.op_config(i) == dom["cfg"])
+    traffic = measured_traffic(args.ops, args.seed, dom["cfg"])
     err_learned, worst_learned = stream_error(torch, ctx, st, plain=False)
     err_plain, worst_plain = stream_error(torch, ctx, st, plain=True)
     print("errors: learned %.3g (worst %.3g)  plain %.3g (worst %.3g)" % (err_learned, worst_learned, err_plain,
@@ -437,8 +463,9 @@ def main():
         "policies": results,
         "groups": groups,
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": MFMA_BF16_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None,
-                     "kernel": "gemm_grouped_kernel cfg %d" % dom["cfg"]},
+                     "unit": "TFLOP/s", "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": traffic,
+                     "kernel": "gemm_grouped_kernel cfg %d" % dom["cfg"], "algorithmic_bytes": dom_bytes,
+                     "launch_ms": round(dom["ms"], 4)},
         "cpu_baseline": cpu,
         "device": info,
     }

@@ -1,0 +1,38 @@
+"""Host-side parts of bench.py: the stream law, the traffic summary parser and the
+one-GPU guard (no GPU needed)."""
+import math
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+import bench  # noqa: E402
+
+
+def test_stream_shapes_law():
+    s = bench.stream_shapes(512, 2024)
+    a = np.array(s)
+    assert a.shape == (512, 3)
+    assert a.min() >= 128 and a.max() <= 4096 and np.all(a % 64 == 0)
+    # log-uniform: the median of each dimension sits near sqrt(128 * 4096) = 724
+    med = np.median(a, axis=0)
+    assert np.all(np.abs(np.log(med / math.sqrt(128 * 4096))) < 0.25)
+    assert bench.stream_shapes(512, 2024) == s
+    assert abs(bench.flops_of(s) / 1e12 - 1.8341) < 1e-4
+
+
+def test_traffic_summary_is_parsed_for_its_workload_only():
+    t = bench.measured_traffic(512, 2024, 1)
+    assert t is not None and 10e9 < t < 20e9
+    assert bench.measured_traffic(512, 2024, 2) is None
+    assert bench.measured_traffic(256, 2024, 1) is None
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
