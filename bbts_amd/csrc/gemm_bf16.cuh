@@ -118,15 +118,183 @@ struct op_view {
   int vecA, vecB;
 };
 
-// Can op use the branch-free FAST path for operand type InT?
+// Can op use the branch-free FAST path for operand type InT?  Besides 16-byte vectors this
+// needs each operand's extent below 2 GiB: the BF16 FAST tile reads through buffer
+// descriptors with 32-bit offsets.
 inline bool fast_ok(const op_view &v, int epv) {
   auto al = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
-  return al(v.A) && al(v.B) && v.K % epv == 0 && v.N % epv == 0 && v.lda % epv == 0 && v.ldb % epv == 0;
+  const int64_t elem = 16 / epv;
+  const int64_t extA = v.M > 0 && v.K > 0 ? ((int64_t)(v.M - 1) * v.lda + v.K) * elem : 0;
+  const int64_t extB = v.K > 0 && v.N > 0 ? ((int64_t)(v.K - 1) * v.ldb + v.N) * elem : 0;
+  return al(v.A) && al(v.B) && v.K % epv == 0 && v.N % epv == 0 && v.lda % epv == 0 && v.ldb % epv == 0 &&
+         extA < (int64_t(1) << 31) && extB < (int64_t(1) << 31);
+}
+
+// ---------------------------------------------------------------------------
+// BF16 operands, FAST shapes: the stream's hot tile.
+//
+// Operands are read through buffer descriptors whose record count is the operand's
+// extent; a lane whose vector lies outside the tile (rows past M, columns past N, k past
+// K) gets offset 0x80000000, which the descriptor's range check turns into zeros with no
+// memory access, so there are no selects on the data path.  In full K stages the stage's
+// k offset is a scalar (the soffset operand), so a stage's loads cost no vector ALU work.
+// Global reads run two stages ahead of the MFMAs through two register sets: the loads of
+// stage k+2 are issued before stage k is computed, and stage k+1, loaded one iteration
+// earlier, is written to the free LDS buffer after it; one barrier per stage.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kOOB = 0x80000000u;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, int64_t bytes) {
+  // raw buffer (stride 0); dword3 0x00020000 is the gfx9 data-format word for untyped access
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)(bytes > 0 ? bytes : 0),
+                                           0x00020000);
+}
+
+__device__ __forceinline__ uint4 bload16(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+
+template <int BM, int BN>
+__device__ __forceinline__ void tile_bf16(const op_view &op, int m0, int n0, uint16_t *smem) {
+  constexpr int BK = 64;
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int FM = WM / 32, FN = WN / 32;
+  constexpr int AE = BM * BK, BE = BN * BK;
+  constexpr int A_PER = BM * (BK / 8) / kThreads;  // 16-byte vectors per lane per stage
+  constexpr int BVPR = BN / 8;                     // vectors per B row (n-contiguous)
+  constexpr int BRS = kThreads / BVPR;             // B rows covered by one pass of the workgroup
+  constexpr int B_PER = BK / BRS;
+  static_assert(A_PER * kThreads == BM * (BK / 8) && B_PER * BRS == BK, "whole vectors per lane");
+
+  uint16_t *As = smem;           // [2][BM][BK]  k-contiguous, swizzled
+  uint16_t *Bs = smem + 2 * AE;  // [2][BK][BN]  n-contiguous, swizzled
+
+  const int M = op.M, N = op.N, K = op.K;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const __amdgpu_buffer_rsrc_t rA = make_rsrc(op.A, ((int64_t)(M - 1) * op.lda + K) * 2);
+  const __amdgpu_buffer_rsrc_t rB = make_rsrc(op.B, ((int64_t)(K - 1) * op.ldb + N) * 2);
+
+  // lane's A vectors: row (tid / 8) + 32 r, chunk tid % 8; B vectors: k row tid / BVPR + BRS r,
+  // chunk tid % BVPR
+  const int ac = tid & 7, arow = tid >> 3;
+  const int bc = tid % BVPR, brow = tid / BVPR;
+  uint32_t va[A_PER], vb[B_PER];
+#pragma unroll
+  for (int r = 0; r < A_PER; ++r) {
+    const int gm = m0 + arow + 32 * r;
+    va[r] = gm < M ? (uint32_t)(((int64_t)gm * op.lda + ac * 8) * 2) : kOOB;
+  }
+  const bool bcol_ok = n0 + bc * 8 < N;
+#pragma unroll
+  for (int r = 0; r < B_PER; ++r)
+    vb[r] = bcol_ok ? (uint32_t)(((int64_t)(brow + BRS * r) * op.ldb + n0 + bc * 8) * 2) : kOOB;
+
+  auto load = [&](uint4 (&ra)[A_PER], uint4 (&rb)[B_PER], int k0) {
+    if (k0 + BK <= K) {
+      const uint32_t sa = (uint32_t)k0 * 2u, sb = (uint32_t)((int64_t)k0 * op.ldb * 2);
+#pragma unroll
+      for (int r = 0; r < A_PER; ++r) ra[r] = bload16(rA, va[r], sa);
+#pragma unroll
+      for (int r = 0; r < B_PER; ++r) rb[r] = bload16(rB, vb[r], sb);
+    } else {
+      const uint32_t sa = (uint32_t)k0 * 2u, sb = (uint32_t)((int64_t)k0 * op.ldb * 2);
+      const bool ak = k0 + ac * 8 < K;
+#pragma unroll
+      for (int r = 0; r < A_PER; ++r) ra[r] = bload16(rA, ak ? va[r] + sa : kOOB, 0u);
+#pragma unroll
+      for (int r = 0; r < B_PER; ++r)
+        rb[r] = bload16(rB, (k0 + brow + BRS * r < K) ? vb[r] + sb : kOOB, 0u);
+    }
+  };
+
+  auto store = [&](const uint4 (&ra)[A_PER], const uint4 (&rb)[B_PER], int buf) {
+    uint16_t *as = As + buf * AE;
+    uint16_t *bs = Bs + buf * BE;
+#pragma unroll
+    for (int r = 0; r < A_PER; ++r) *reinterpret_cast<uint4 *>(&as[sw<BK>(arow + 32 * r, ac * 8)]) = ra[r];
+#pragma unroll
+    for (int r = 0; r < B_PER; ++r) *reinterpret_cast<uint4 *>(&bs[swn<BN>(brow + BRS * r, bc * 8)]) = rb[r];
+  };
+
+  f16v acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
+
+  const int half = lane >> 5, col = lane & 31;
+  auto compute = [&](int buf) {
+    const uint16_t *as = As + buf * AE;
+    const uint16_t *bs = Bs + buf * BE;
+#pragma unroll
+    for (int kk = 0; kk < BK / 16; ++kk) {
+      const int k = kk * 16 + half * 8;
+      bf16x8 af[FM], bfr[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+        af[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&as[sw<BK>(wm * WM + i * 32 + col, k)]));
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int g = lane >> 4, i = lane & 15;
+        const int kr = kk * 16 + 8 * (g >> 1) + (i >> 2);
+        const int nc = wn * WN + j * 32 + 16 * (g & 1) + 4 * (i & 3);
+        struct {
+          v4s a, b;
+        } both{read_tr16(&bs[swn<BN>(kr, nc)]), read_tr16(&bs[swn<BN>(kr + 4, nc)])};
+        bfr[j] = __builtin_bit_cast(bf16x8, both);
+      }
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    }
+  };
+
+  uint4 a0[A_PER], b0[B_PER], a1[A_PER], b1[B_PER];
+  const int nk = (K + BK - 1) / BK;
+  __syncthreads();  // LDS may still be read by the previous tile of a persistent loop
+  if (nk > 0) load(a0, b0, 0);
+  if (nk > 1) load(a1, b1, BK);
+  if (nk > 0) store(a0, b0, 0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; kt += 2) {
+    if (kt + 2 < nk) load(a0, b0, (kt + 2) * BK);
+    compute(0);
+    if (kt + 1 < nk) store(a1, b1, 1);
+    __syncthreads();
+    if (kt + 1 >= nk) break;
+    if (kt + 3 < nk) load(a1, b1, (kt + 3) * BK);
+    compute(1);
+    if (kt + 2 < nk) store(a0, b0, 0);
+    __syncthreads();
+  }
+
+  const float alpha = op.alpha;
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int gn = n0 + wn * WN + j * 32 + col;
+      if (gn >= N) continue;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int gm = m0 + wm * WM + i * 32 + (v >> 2) * 8 + half * 4 + (v & 3);
+        if (gm < M) op.C[(int64_t)gm * op.ldc + gn] = alpha * acc[i][j][v];
+      }
+    }
 }
 
 // Compute one BM x BN tile at (m0, n0).  smem must hold smem_layout<BM,BN,S>::bytes.
 template <int BM, int BN, typename InT, int S, int T, bool FAST>
 __device__ __forceinline__ void tile(const op_view &op, int m0, int n0, uint16_t *smem) {
+  if constexpr (FAST && S == 1 && T == 1 && sizeof(InT) == 2 && BM * BN <= 128 * 128) {
+    tile_bf16<BM, BN>(op, m0, n0, smem);
+    return;
+  }
   constexpr int BK = stage_k<BM, BN, S>::value;
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 32, FN = WN / 32;

@@ -43,8 +43,17 @@ __device__ inline void map_tile2(int lin, int tiles_m, int tiles_n, int *tm, int
   *tn = in_group / gsize;
 }
 
+// Waves per SIMD the register allocator must leave room for (BF16-operand tiles; the
+// split-precision tiles keep the allocator's own choice).
+template <int BM, int BN, typename InT, int S>
+struct waves_target {
+  static constexpr int value =
+      (sizeof(InT) == 2 && S == 1) ? (BM * BN >= 256 * 128 ? 1 : (BM * BN >= 128 * 128 ? 2 : (BM * BN >= 128 * 64 ? 3 : 4)))
+                                   : 1;
+};
+
 template <int BM, int BN, typename InT, int S, int T, bool FAST>
-__global__ __launch_bounds__(256) void gemm_single_kernel(op_view op, int64_t sa, int64_t sb, int64_t sc, int tiles_m,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_target<BM, BN, InT, S>::value))) void gemm_single_kernel(op_view op, int64_t sa, int64_t sb, int64_t sc, int tiles_m,
                                                           int tiles_n, uint32_t *meta_dst, uint32_t w0, uint32_t w1) {
   extern __shared__ __align__(16) uint16_t smem[];
   if (meta_dst != nullptr && blockIdx.x == 0 && blockIdx.z == 0 && threadIdx.x == 0) {
@@ -63,7 +72,7 @@ __global__ __launch_bounds__(256) void gemm_single_kernel(op_view op, int64_t sa
 // Queue x of nq holds items [qstart[x], qstart[x+1]); workgroup w serves queue w % nq
 // (the grid is a multiple of nq), with its own ticket counter at counters[64 x].
 template <int BM, int BN, typename InT, int S, int T, bool FAST>
-__global__ __launch_bounds__(256) void gemm_grouped_kernel(const gop *__restrict__ ops, const int2 *__restrict__ items,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_target<BM, BN, InT, S>::value))) void gemm_grouped_kernel(const gop *__restrict__ ops, const int2 *__restrict__ items,
                                                            const int *__restrict__ qstart, int nq,
                                                            unsigned *counters) {
   extern __shared__ __align__(16) uint16_t smem[];
