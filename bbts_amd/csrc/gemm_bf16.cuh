@@ -119,15 +119,16 @@ struct op_view {
 };
 
 // Can op use the branch-free FAST path for operand type InT?  Besides 16-byte vectors this
-// needs each operand's extent below 2 GiB: the BF16 FAST tile reads through buffer
-// descriptors with 32-bit offsets.
+// needs each operand's extent below 2 GiB: the BF16 FAST tile reads A and B and writes C
+// through buffer descriptors with 32-bit offsets.
 inline bool fast_ok(const op_view &v, int epv) {
   auto al = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
   const int64_t elem = 16 / epv;
   const int64_t extA = v.M > 0 && v.K > 0 ? ((int64_t)(v.M - 1) * v.lda + v.K) * elem : 0;
   const int64_t extB = v.K > 0 && v.N > 0 ? ((int64_t)(v.K - 1) * v.ldb + v.N) * elem : 0;
+  const int64_t extC = v.M > 0 && v.N > 0 ? ((int64_t)(v.M - 1) * v.ldc + v.N) * 4 : 0;
   return al(v.A) && al(v.B) && v.K % epv == 0 && v.N % epv == 0 && v.lda % epv == 0 && v.ldb % epv == 0 &&
-         extA < (int64_t(1) << 31) && extB < (int64_t(1) << 31);
+         extA < (int64_t(1) << 31) && extB < (int64_t(1) << 31) && extC < (int64_t(1) << 31);
 }
 
 // ---------------------------------------------------------------------------
@@ -136,11 +137,13 @@ inline bool fast_ok(const op_view &v, int epv) {
 // Operands are read through buffer descriptors whose record count is the operand's
 // extent; a lane whose vector lies outside the tile (rows past M, columns past N, k past
 // K) gets offset 0x80000000, which the descriptor's range check turns into zeros with no
-// memory access, so there are no selects on the data path.  In full K stages the stage's
-// k offset is a scalar (the soffset operand), so a stage's loads cost no vector ALU work.
-// Global reads run two stages ahead of the MFMAs through two register sets: the loads of
-// stage k+2 are issued before stage k is computed, and stage k+1, loaded one iteration
-// earlier, is written to the free LDS buffer after it; one barrier per stage.
+// memory access, so there are no selects on the data path.  The stage's k offset is a
+// scalar (the soffset operand); per stage a lane only picks its offset or the out-of-range
+// one.  Global reads run two stages ahead of the MFMAs through two register sets: the
+// loads of stage k+2 are issued before stage k is computed, and stage k+1, loaded one
+// iteration earlier, is written to the free LDS buffer after it; one barrier per stage.
+// The K loop runs in pairs of stages with no conditional loads or stores, so the wait
+// before an LDS write leaves the newest stage's loads in flight.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kOOB = 0x80000000u;
 
@@ -190,22 +193,17 @@ __device__ __forceinline__ void tile_bf16(const op_view &op, int m0, int n0, uin
   for (int r = 0; r < B_PER; ++r)
     vb[r] = bcol_ok ? (uint32_t)(((int64_t)(brow + BRS * r) * op.ldb + n0 + bc * 8) * 2) : kOOB;
 
+  // Straight-line stage load (no branch, so the wait before each LDS write can leave the
+  // next stage's loads in flight): lanes past K, like lanes past M or N, get the
+  // out-of-range offset. Stages at or beyond K read nothing and yield zeros.
   auto load = [&](uint4 (&ra)[A_PER], uint4 (&rb)[B_PER], int k0) {
-    if (k0 + BK <= K) {
-      const uint32_t sa = (uint32_t)k0 * 2u, sb = (uint32_t)((int64_t)k0 * op.ldb * 2);
+    const int lim = K - k0;
+    const uint32_t sa = (uint32_t)k0 * 2u, sb = (uint32_t)((int64_t)(k0 > 0 ? k0 : 0) * op.ldb * 2);
+    const bool ak = ac * 8 < lim;
 #pragma unroll
-      for (int r = 0; r < A_PER; ++r) ra[r] = bload16(rA, va[r], sa);
+    for (int r = 0; r < A_PER; ++r) ra[r] = bload16(rA, ak ? va[r] : kOOB, sa);
 #pragma unroll
-      for (int r = 0; r < B_PER; ++r) rb[r] = bload16(rB, vb[r], sb);
-    } else {
-      const uint32_t sa = (uint32_t)k0 * 2u, sb = (uint32_t)((int64_t)k0 * op.ldb * 2);
-      const bool ak = k0 + ac * 8 < K;
-#pragma unroll
-      for (int r = 0; r < A_PER; ++r) ra[r] = bload16(rA, ak ? va[r] + sa : kOOB, 0u);
-#pragma unroll
-      for (int r = 0; r < B_PER; ++r)
-        rb[r] = bload16(rB, (k0 + brow + BRS * r < K) ? vb[r] + sb : kOOB, 0u);
-    }
+    for (int r = 0; r < B_PER; ++r) rb[r] = bload16(rB, (brow + BRS * r < lim) ? vb[r] : kOOB, sb);
   };
 
   auto store = [&](const uint4 (&ra)[A_PER], const uint4 (&rb)[B_PER], int buf) {
@@ -257,33 +255,39 @@ __device__ __forceinline__ void tile_bf16(const op_view &op, int m0, int n0, uin
   uint4 a0[A_PER], b0[B_PER], a1[A_PER], b1[B_PER];
   const int nk = (K + BK - 1) / BK;
   __syncthreads();  // LDS may still be read by the previous tile of a persistent loop
-  if (nk > 0) load(a0, b0, 0);
-  if (nk > 1) load(a1, b1, BK);
-  if (nk > 0) store(a0, b0, 0);
+  load(a0, b0, 0);
+  load(a1, b1, BK);
+  store(a0, b0, 0);
   __syncthreads();
-  for (int kt = 0; kt < nk; kt += 2) {
-    if (kt + 2 < nk) load(a0, b0, (kt + 2) * BK);
+  // stages in pairs: set 0 / buffer 0 hold even stages, set 1 / buffer 1 odd ones
+  int kt = 0;
+  for (; kt + 1 < nk; kt += 2) {
+    load(a0, b0, (kt + 2) * BK);
+    __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of this stage's LDS writes
     compute(0);
-    if (kt + 1 < nk) store(a1, b1, 1);
+    store(a1, b1, 1);
     __syncthreads();
-    if (kt + 1 >= nk) break;
-    if (kt + 3 < nk) load(a1, b1, (kt + 3) * BK);
+    load(a1, b1, (kt + 3) * BK);
+    __builtin_amdgcn_sched_barrier(0);
     compute(1);
-    if (kt + 2 < nk) store(a0, b0, 0);
+    store(a0, b0, 0);
     __syncthreads();
   }
+  if (kt < nk) compute(0);  // odd stage count: the last stage sits in buffer 0
 
+  // epilogue through a descriptor over C: elements outside the M x N block are dropped
+  const __amdgpu_buffer_rsrc_t rC = make_rsrc(op.C, ((int64_t)(M - 1) * op.ldc + N) * 4);
   const float alpha = op.alpha;
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
       const int gn = n0 + wn * WN + j * 32 + col;
-      if (gn >= N) continue;
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int gm = m0 + wm * WM + i * 32 + (v >> 2) * 8 + half * 4 + (v & 3);
-        if (gm < M) op.C[(int64_t)gm * op.ldc + gn] = alpha * acc[i][j][v];
+        const uint32_t off = (gm < M && gn < N) ? (uint32_t)(((int64_t)gm * op.ldc + gn) * 4) : kOOB;
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, alpha * acc[i][j][v]), rC, off, 0u, 0);
       }
     }
 }
