@@ -22,8 +22,11 @@
  *                           (libraries/cutensor/contraction.h:131-202), numpy.einsum
  *                           in FP64, GEMM special cases
  *   orc_rng_* ............. distribution moments; exact closed form of the counter mix
- * The "parity unpinned" items are listed in the Python wrapper's header.
+ * What pins each function is listed in the header of oracle/oracle.py.
  */
+#ifndef _GNU_SOURCE
+#define _GNU_SOURCE /* RTLD_DEFAULT */
+#endif
 #include <dlfcn.h>
 #include <math.h>
 #include <stddef.h>
@@ -234,6 +237,13 @@ static void orc_resolve_sgemm(void) {
     if (!h) continue;
     void *f = dlsym(h, "cblas_sgemm");
     if (f) {
+      /* MKL's default Intel threading layer expects libiomp5's OpenMP entry points; in a
+       * process that already has GNU OpenMP (libgomp) in its global scope, as any process
+       * that imported torch has, it then computes wrong products.  The GNU layer runs on the
+       * libgomp that is already there.  (MKL_THREADING_GNU = 3 in mkl_service.h.) */
+      typedef int (*orc_set_layer_t)(int);
+      orc_set_layer_t set_layer = (orc_set_layer_t)dlsym(h, "MKL_Set_Threading_Layer");
+      if (set_layer && dlsym(RTLD_DEFAULT, "GOMP_parallel")) set_layer(3);
       orc_sgemm_fn = (orc_cblas_sgemm_t)f;
       orc_set_threads_fn = (orc_set_threads_t)dlsym(h, "MKL_Set_Num_Threads");
       if (!orc_set_threads_fn) orc_set_threads_fn = (orc_set_threads_t)dlsym(h, "openblas_set_num_threads");

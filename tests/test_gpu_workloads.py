@@ -1,0 +1,192 @@
+"""Parity at the workloads' full sizes.
+
+* bmm, 4096 x 4096 in a 4 x 4 grid of 1024^2 blocks, U[-1, 1] blocks from the uniform ud,
+  through the runtime's command runner (64 APPLY matrix_mult + 16 REDUCE matrix_add, the
+  product temporaries deleted), as applications/bmm/generate_bmm.cc:232-266 builds it.
+  Every output block against an FP64 product of the same values (<= 5e-6 relative
+  Frobenius), and one output block against the oracle's CPU ud path (MKL sgemm per block
+  then the left-fold reduce, dense_matrix_mult.cc:82 / local_reduce_op.cc:23-91).
+* cpmm, A 8192 x 16384 and B 16384 x 8192 N(0, 1) in 2048^2 blocks: 128 block products by
+  the FP32 drop-in, each against its FP64 product on the GPU (<= 3e-6); 16 k = 8 reduces,
+  each bit-identical to the left fold of its product blocks; one 2048^2 block broadcast to
+  32 positions, each destination byte-identical to the source (header and payload).
+* contraction abcd,cdef->abef, a = b = e = f = 96, c = d = 128, N(0, 1), tiled 8 ways in a,
+  c and e (512 block contractions of M = N = 1152, K = 2048 through cutensor-format blocks,
+  8-way reduces over the c blocks), assembled and checked against the untiled FP64
+  contraction on the GPU: <= 5e-6 for FP32 and BF16x6, <= 1e-5 for BF16x3.
+"""
+import ctypes
+import math
+import struct
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _copy(ctx, src, dst, nbytes):
+    from bbts_amd import call
+    arr = (ctypes.c_void_p * 1)(dst)
+    call("bbts_hip_copy_n", ctx.h, src, nbytes, arr, 1, None)
+
+
+def _residual(ctx, out2, M, N, K, A, lda, B, ldb, C, ldc):
+    from bbts_amd import call
+    call("bbts_hip_residual_f64", ctx.h, M, N, K, A, lda, 0, B, ldb, 0, C, ldc, out2.data_ptr(), None)
+    e2, r2 = out2.cpu().numpy()
+    return math.sqrt(e2 / r2)
+
+
+def test_bmm_4096_in_1024_blocks(ctx, orc, torch_cuda):
+    from bbts_amd.runtime import Runtime
+    n, bs, seed = 4, 1024, 7000
+    rt = Runtime(ctx)
+    uni, mult, add = rt.impl("uniform", False), rt.impl("matrix_mult", True), rt.impl("matrix_add", True)
+    tid = lambda m, i, j: m * 10000 + i * 100 + j
+    for i in range(n):
+        for j in range(n):
+            rt.apply(uni, [], [tid(1, i, j)], [bs, bs, -1.0, 1.0, seed + tid(1, i, j)])
+            rt.apply(uni, [], [tid(2, i, j)], [bs, bs, -1.0, 1.0, seed + tid(2, i, j)])
+    temps = []
+    for i in range(n):
+        for j in range(n):
+            prods = [tid(5, i * n + j, k) for k in range(n)]
+            for k in range(n):
+                rt.apply(mult, [tid(1, i, k), tid(2, k, j)], [prods[k]])
+            rt.reduce(add, prods, tid(3, i, j))
+            temps += prods
+    rt.delete(temps)
+    print("bmm 4096: running 64 products + 16 reduces", flush=True)
+    rt.run(0, 0)
+    assert rt.num_tensors() == 3 * n * n
+
+    blk = {(m, i, j): orc.uniform(bs * bs, seed + tid(m, i, j), -1.0, 1.0).reshape(bs, bs)
+           for m in (1, 2) for i in range(n) for j in range(n)}
+    A = np.block([[blk[1, i, k].astype(np.float64) for k in range(n)] for i in range(n)])
+    B = np.block([[blk[2, k, j].astype(np.float64) for j in range(n)] for k in range(n)])
+    C64 = A @ B
+    for i in range(n):
+        for j in range(n):
+            meta, nbytes = rt.meta(tid(3, i, j))
+            assert struct.unpack_from("<iII", meta, 0) == (0, bs, bs) and nbytes == 260 + 4 * bs * bs
+            got = rt.get_dense(tid(3, i, j)).astype(np.float64)
+            ref = C64[i * bs:(i + 1) * bs, j * bs:(j + 1) * bs]
+            assert np.linalg.norm(got - ref) <= 5e-6 * np.linalg.norm(ref), (i, j)
+    # the reference's CPU ud path on output block (0, 0): per-block sgemm, then the left fold
+    prods = [orc.dense_mult_f32(blk[1, 0, k], blk[2, k, 0]) for k in range(n)]
+    cpu = orc.local_reduce(prods).astype(np.float64)
+    got = rt.get_dense(tid(3, 0, 0)).astype(np.float64)
+    assert np.linalg.norm(got - cpu) <= 5e-6 * np.linalg.norm(cpu)
+    rt.close()
+
+
+def test_cpmm_2048_blocks_reduce_and_broadcast(ctx, orc, torch_cuda):
+    torch = torch_cuda
+    bs, I, Kb, J = 2048, 4, 8, 4
+    nb = 4 * bs * bs
+    out2 = torch.zeros(2, dtype=torch.float64, device="cuda")
+    tmp = torch.empty(bs * bs, dtype=torch.float32, device="cuda")
+
+    def normal_block(seed):
+        b = ctx.dense(bs, bs)
+        ctx.gen_normal(tmp, seed=seed)
+        _copy(ctx, tmp.data_ptr(), b.data_ptr, nb)
+        return b
+
+    A = {(i, k): normal_block(100 + i * Kb + k) for i in range(I) for k in range(Kb)}
+    B = {(k, j): normal_block(200 + k * J + j) for k in range(Kb) for j in range(J)}
+    P = {(i, j, k): ctx.dense(bs, bs) for i in range(I) for j in range(J) for k in range(Kb)}
+    print("cpmm: 128 products of 2048^3", flush=True)
+    for (i, j, k), p in P.items():
+        ctx.matmul(A[i, k], B[k, j], p)
+    ctx.sync()
+    worst = 0.0
+    for (i, j, k), p in P.items():
+        assert p.dense_shape() == (bs, bs)
+        e = _residual(ctx, out2, bs, bs, bs, A[i, k].data_ptr, bs, B[k, j].data_ptr, bs, p.data_ptr, bs)
+        worst = max(worst, e)
+    print("cpmm products: worst relative error %.3g" % worst, flush=True)
+    assert worst <= 3e-6
+
+    C = {(i, j): ctx.dense(bs, bs) for i in range(I) for j in range(J)}
+    for (i, j), c in C.items():
+        ctx.reduce([P[i, j, k] for k in range(Kb)], c)
+    ctx.sync()
+    acc = torch.empty(bs * bs, dtype=torch.float32, device="cuda")
+    got = torch.empty_like(acc)
+    for (i, j), c in C.items():
+        _copy(ctx, P[i, j, 0].data_ptr, acc.data_ptr(), nb)
+        for k in range(1, Kb):
+            _copy(ctx, P[i, j, k].data_ptr, tmp.data_ptr(), nb)
+            torch.cuda.synchronize()
+            acc += tmp  # ((p0 + p1) + p2) + ... in FP32, one IEEE add per element
+        _copy(ctx, c.data_ptr, got.data_ptr(), nb)
+        torch.cuda.synchronize()
+        assert torch.equal(got, acc), (i, j)
+
+    src = C[0, 0]
+    dst = [ctx.dense(bs, bs) for _ in range(32)]
+    ctx.broadcast(src, dst)
+    ctx.sync()
+    hdr = src.download_bytes(260).tobytes()
+    _copy(ctx, src.data_ptr, acc.data_ptr(), nb)
+    for d in dst:
+        assert d.download_bytes(260).tobytes() == hdr
+        _copy(ctx, d.data_ptr, got.data_ptr(), nb)
+        torch.cuda.synchronize()
+        assert torch.equal(got, acc)
+    for b in list(A.values()) + list(B.values()) + list(P.values()) + list(C.values()) + dst:
+        b.free()
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 5e-6), (2, 1e-5), (3, 5e-6)])
+def test_contraction_abcd_cdef_tiled(ctx, orc, torch_cuda, prec, tol):
+    torch = torch_cuda
+    from bbts_amd import call
+    a = b = e = f = 96
+    c = d = 128
+    T = 8
+    ta, tc, te = a // T, c // T, e // T
+    # column-major [a,b,c,d] is row-major [d,c,b,a]
+    L = torch.empty((d, c, b, a), dtype=torch.float32, device="cuda")
+    R = torch.empty((f, e, d, c), dtype=torch.float32, device="cuda")
+    ctx.gen_normal(L, seed=51)
+    ctx.gen_normal(R, seed=52)
+    O = torch.full((f, e, b, a), float("nan"), dtype=torch.float32, device="cuda")
+    p = orc.contraction_params([0, 1, 2, 3], [2, 3, 4, 5], [0, 1, 4, 5], 1.0)
+
+    def to_block(t, dims):
+        blk = ctx.cutensor(dims)
+        src = t.contiguous()
+        _copy(ctx, src.data_ptr(), blk.data_ptr, 4 * src.numel())
+        torch.cuda.synchronize()
+        return blk
+
+    lhs = {(x, z): to_block(L[:, z * tc:(z + 1) * tc, :, x * ta:(x + 1) * ta], [ta, b, tc, d])
+           for x in range(T) for z in range(T)}
+    rhs = {(z, y): to_block(R[:, y * te:(y + 1) * te, :, z * tc:(z + 1) * tc], [tc, d, te, f])
+           for z in range(T) for y in range(T)}
+    n_out = ta * b * te * f
+    part = [ctx.cutensor([ta, b, te, f]) for _ in range(T)]
+    out = ctx.cutensor([ta, b, te, f])
+    ob = torch.empty(n_out, dtype=torch.float32, device="cuda")
+    print("contraction: 512 block contractions, precision %d" % prec, flush=True)
+    for x in range(T):
+        for y in range(T):
+            for z in range(T):
+                ctx.contraction(lhs[x, z], rhs[z, y], part[z], p, precision=prec)
+            arr = (ctypes.c_void_p * T)(*[q.data_ptr for q in part])
+            call("bbts_hip_reduce_f32", ctx.h, arr, T, out.data_ptr, n_out, None)
+            _copy(ctx, out.data_ptr, ob.data_ptr(), 4 * n_out)
+            torch.cuda.synchronize()
+            O[:, y * te:(y + 1) * te, :, x * ta:(x + 1) * ta] = ob.view(f, te, b, ta)
+    assert out.cu_shape() == (ta, b, te, f)
+    torch.cuda.synchronize()
+    # O[ef][ab] = sum_cd R[ef][cd] L[cd][ab] as row-major matrices
+    out2 = torch.zeros(2, dtype=torch.float64, device="cuda")
+    err = _residual(ctx, out2, e * f, a * b, c * d, R.data_ptr(), c * d, L.data_ptr(), a * b, O.data_ptr(), a * b)
+    print("tiled contraction precision %d: relative error %.3g" % (prec, err), flush=True)
+    assert err <= tol
+    for blk in list(lhs.values()) + list(rhs.values()) + part + [out]:
+        blk.free()

@@ -163,6 +163,21 @@ def test_dense_mult_f32_matches_f64(orc):
     assert np.linalg.norm(c - ref) / np.linalg.norm(ref) < 1e-6
 
 
+def test_dense_mult_f32_threaded_after_torch(orc):
+    # a process that imported torch has GNU OpenMP loaded; the BLAS provider must still give
+    # the right product on its multithreaded path (blocks large enough to use threads)
+    import torch
+    x = torch.ones(256, 256)
+    assert float((x @ x)[0, 0]) == 256.0
+    orc.sgemm_threads(4)
+    n = 1024
+    a = orc.uniform(n * n, 21, -1.0, 1.0).reshape(n, n)
+    b = orc.uniform(n * n, 22, -1.0, 1.0).reshape(n, n)
+    c = orc.dense_mult_f32(a, b)
+    ref = a.astype(np.float64) @ b.astype(np.float64)
+    assert np.linalg.norm(c - ref) / np.linalg.norm(ref) < 1e-6
+
+
 def test_dgemm(orc):
     a = np.random.default_rng(4).standard_normal((9, 7))
     b = np.random.default_rng(5).standard_normal((7, 11))
