@@ -132,7 +132,7 @@ inline bool fast_ok(const op_view &v, int epv) {
 }
 
 // ---------------------------------------------------------------------------
-// BF16 operands, FAST shapes: the stream's hot tile.
+// FAST shapes (BF16 operands, and FP32 operands split into BF16 slices): the hot tile.
 //
 // Operands are read through buffer descriptors whose record count is the operand's
 // extent; a lane whose vector lies outside the tile (rows past M, columns past N, k past
@@ -157,62 +157,102 @@ __device__ __forceinline__ uint4 bload16(__amdgpu_buffer_rsrc_t r, uint32_t voff
   return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
 }
 
-template <int BM, int BN>
-__device__ __forceinline__ void tile_bf16(const op_view &op, int m0, int n0, uint16_t *smem) {
-  constexpr int BK = 64;
+template <int BM, int BN, typename InT, int S, int T>
+__device__ __forceinline__ void tile_fast(const op_view &op, int m0, int n0, uint16_t *smem) {
+  constexpr int BK = stage_k<BM, BN, S>::value;
+  constexpr int ES = (int)sizeof(InT);               // bytes per operand element
+  constexpr int EPV = 16 / ES;                       // elements per 16-byte vector
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 32, FN = WN / 32;
-  constexpr int AE = BM * BK, BE = BN * BK;
-  constexpr int A_PER = BM * (BK / 8) / kThreads;  // 16-byte vectors per lane per stage
-  constexpr int BVPR = BN / 8;                     // vectors per B row (n-contiguous)
-  constexpr int BRS = kThreads / BVPR;             // B rows covered by one pass of the workgroup
+  constexpr int AE = BM * BK, BE = BN * BK;          // BF16 elements of one slice plane
+  constexpr int VPR = BK / EPV;                      // vectors per A row (k-contiguous)
+  constexpr int ARS = kThreads / VPR;                // A rows covered by one pass of the workgroup
+  constexpr int A_PER = BM / ARS;
+  constexpr int BVPR = BN / EPV;                     // vectors per B row (n-contiguous)
+  constexpr int BRS = kThreads / BVPR;               // B rows covered by one pass
   constexpr int B_PER = BK / BRS;
-  static_assert(A_PER * kThreads == BM * (BK / 8) && B_PER * BRS == BK, "whole vectors per lane");
+  static_assert(A_PER * ARS == BM && B_PER * BRS == BK, "whole vectors per lane");
 
-  uint16_t *As = smem;           // [2][BM][BK]  k-contiguous, swizzled
-  uint16_t *Bs = smem + 2 * AE;  // [2][BK][BN]  n-contiguous, swizzled
+  uint16_t *As = smem;               // [2][S][BM][BK]  k-contiguous, swizzled
+  uint16_t *Bs = smem + 2 * S * AE;  // [2][S][BK][BN]  n-contiguous, swizzled
 
   const int M = op.M, N = op.N, K = op.K;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const __amdgpu_buffer_rsrc_t rA = make_rsrc(op.A, ((int64_t)(M - 1) * op.lda + K) * 2);
-  const __amdgpu_buffer_rsrc_t rB = make_rsrc(op.B, ((int64_t)(K - 1) * op.ldb + N) * 2);
+  const __amdgpu_buffer_rsrc_t rA = make_rsrc(op.A, ((int64_t)(M - 1) * op.lda + K) * ES);
+  const __amdgpu_buffer_rsrc_t rB = make_rsrc(op.B, ((int64_t)(K - 1) * op.ldb + N) * ES);
 
-  // lane's A vectors: row (tid / 8) + 32 r, chunk tid % 8; B vectors: k row tid / BVPR + BRS r,
-  // chunk tid % BVPR
-  const int ac = tid & 7, arow = tid >> 3;
+  // lane's A vectors: row tid / VPR + ARS r, vector tid % VPR; B vectors: k row tid / BVPR
+  // + BRS r, vector tid % BVPR
+  const int ac = tid % VPR, arow = tid / VPR;
   const int bc = tid % BVPR, brow = tid / BVPR;
   uint32_t va[A_PER], vb[B_PER];
 #pragma unroll
   for (int r = 0; r < A_PER; ++r) {
-    const int gm = m0 + arow + 32 * r;
-    va[r] = gm < M ? (uint32_t)(((int64_t)gm * op.lda + ac * 8) * 2) : kOOB;
+    const int gm = m0 + arow + ARS * r;
+    va[r] = gm < M ? (uint32_t)(((int64_t)gm * op.lda + ac * EPV) * ES) : kOOB;
   }
-  const bool bcol_ok = n0 + bc * 8 < N;
+  const bool bcol_ok = n0 + bc * EPV < N;
 #pragma unroll
   for (int r = 0; r < B_PER; ++r)
-    vb[r] = bcol_ok ? (uint32_t)(((int64_t)(brow + BRS * r) * op.ldb + n0 + bc * 8) * 2) : kOOB;
+    vb[r] = bcol_ok ? (uint32_t)(((int64_t)(brow + BRS * r) * op.ldb + n0 + bc * EPV) * ES) : kOOB;
 
   // Straight-line stage load (no branch, so the wait before each LDS write can leave the
   // next stage's loads in flight): lanes past K, like lanes past M or N, get the
   // out-of-range offset. Stages at or beyond K read nothing and yield zeros.
   auto load = [&](uint4 (&ra)[A_PER], uint4 (&rb)[B_PER], int k0) {
     const int lim = K - k0;
-    const uint32_t sa = (uint32_t)k0 * 2u, sb = (uint32_t)((int64_t)(k0 > 0 ? k0 : 0) * op.ldb * 2);
-    const bool ak = ac * 8 < lim;
+    const uint32_t sa = (uint32_t)k0 * (uint32_t)ES;
+    const uint32_t sb = (uint32_t)((int64_t)(k0 > 0 ? k0 : 0) * op.ldb * ES);
+    const bool ak = ac * EPV < lim;
 #pragma unroll
     for (int r = 0; r < A_PER; ++r) ra[r] = bload16(rA, ak ? va[r] : kOOB, sa);
 #pragma unroll
     for (int r = 0; r < B_PER; ++r) rb[r] = bload16(rB, (brow + BRS * r < lim) ? vb[r] : kOOB, sb);
   };
 
+  // BF16 operands go to LDS as loaded; FP32 operands are split into S BF16 slice planes
   auto store = [&](const uint4 (&ra)[A_PER], const uint4 (&rb)[B_PER], int buf) {
-    uint16_t *as = As + buf * AE;
-    uint16_t *bs = Bs + buf * BE;
+    uint16_t *as = As + buf * S * AE;
+    uint16_t *bs = Bs + buf * S * BE;
 #pragma unroll
-    for (int r = 0; r < A_PER; ++r) *reinterpret_cast<uint4 *>(&as[sw<BK>(arow + 32 * r, ac * 8)]) = ra[r];
+    for (int r = 0; r < A_PER; ++r) {
+      const int row = arow + ARS * r;
+      if constexpr (ES == 2) {
+        *reinterpret_cast<uint4 *>(&as[sw<BK>(row, ac * 8)]) = ra[r];
+      } else {
+        const float *f = reinterpret_cast<const float *>(&ra[r]);
+        uint16_t sl[4][S];
 #pragma unroll
-    for (int r = 0; r < B_PER; ++r) *reinterpret_cast<uint4 *>(&bs[swn<BN>(brow + BRS * r, bc * 8)]) = rb[r];
+        for (int e = 0; e < 4; ++e) split<S>(f[e], sl[e]);
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+          uint2 v;
+          v.x = (uint32_t)sl[0][q] | ((uint32_t)sl[1][q] << 16);
+          v.y = (uint32_t)sl[2][q] | ((uint32_t)sl[3][q] << 16);
+          *reinterpret_cast<uint2 *>(&as[q * AE + sw<BK>(row, ac * 4)]) = v;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < B_PER; ++r) {
+      const int krow = brow + BRS * r;
+      if constexpr (ES == 2) {
+        *reinterpret_cast<uint4 *>(&bs[swn<BN>(krow, bc * 8)]) = rb[r];
+      } else {
+        const float *f = reinterpret_cast<const float *>(&rb[r]);
+        uint16_t sl[4][S];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) split<S>(f[e], sl[e]);
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+          uint2 v;
+          v.x = (uint32_t)sl[0][q] | ((uint32_t)sl[1][q] << 16);
+          v.y = (uint32_t)sl[2][q] | ((uint32_t)sl[3][q] << 16);
+          *reinterpret_cast<uint2 *>(&bs[q * BE + swn<BN>(krow, bc * 4)]) = v;
+        }
+      }
+    }
   };
 
   f16v acc[FM][FN];
@@ -225,30 +265,44 @@ __device__ __forceinline__ void tile_bf16(const op_view &op, int m0, int n0, uin
 
   const int half = lane >> 5, col = lane & 31;
   auto compute = [&](int buf) {
-    const uint16_t *as = As + buf * AE;
-    const uint16_t *bs = Bs + buf * BE;
+    const uint16_t *as = As + buf * S * AE;
+    const uint16_t *bs = Bs + buf * S * BE;
 #pragma unroll
     for (int kk = 0; kk < BK / 16; ++kk) {
       const int k = kk * 16 + half * 8;
-      bf16x8 af[FM], bfr[FN];
+      bf16x8 af[S][FM], bfr[S][FN];
 #pragma unroll
-      for (int i = 0; i < FM; ++i)
-        af[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&as[sw<BK>(wm * WM + i * 32 + col, k)]));
+      for (int q = 0; q < S; ++q) {
 #pragma unroll
-      for (int j = 0; j < FN; ++j) {
-        const int g = lane >> 4, i = lane & 15;
-        const int kr = kk * 16 + 8 * (g >> 1) + (i >> 2);
-        const int nc = wn * WN + j * 32 + 16 * (g & 1) + 4 * (i & 3);
-        struct {
-          v4s a, b;
-        } both{read_tr16(&bs[swn<BN>(kr, nc)]), read_tr16(&bs[swn<BN>(kr + 4, nc)])};
-        bfr[j] = __builtin_bit_cast(bf16x8, both);
+        for (int i = 0; i < FM; ++i)
+          af[q][i] = __builtin_bit_cast(
+              bf16x8, *reinterpret_cast<const uint4 *>(&as[q * AE + sw<BK>(wm * WM + i * 32 + col, k)]));
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          // MFMA B operand of lane l: B[kk*16 + 8*(l/32) + t][n = l % 32], t = 0..7, read as two
+          // transposing 4-row reads of the 16-lane group g = l/16
+          const int g = lane >> 4, i = lane & 15;
+          const int kr = kk * 16 + 8 * (g >> 1) + (i >> 2);
+          const int nc = wn * WN + j * 32 + 16 * (g & 1) + 4 * (i & 3);
+          struct {
+            v4s a, b;
+          } both{read_tr16(&bs[q * BE + swn<BN>(kr, nc)]), read_tr16(&bs[q * BE + swn<BN>(kr + 4, nc)])};
+          bfr[q][j] = __builtin_bit_cast(bf16x8, both);
+        }
       }
+      // small terms first, the leading product last
 #pragma unroll
-      for (int i = 0; i < FM; ++i)
+      for (int d = T - 1; d >= 0; --d)
 #pragma unroll
-        for (int j = 0; j < FN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+        for (int sa = 0; sa < S; ++sa) {
+          const int sb = d - sa;
+          if (sb < 0 || sb >= S) continue;
+#pragma unroll
+          for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[sa][i], bfr[sb][j], acc[i][j], 0, 0, 0);
+        }
     }
   };
 
@@ -295,8 +349,8 @@ __device__ __forceinline__ void tile_bf16(const op_view &op, int m0, int n0, uin
 // Compute one BM x BN tile at (m0, n0).  smem must hold smem_layout<BM,BN,S>::bytes.
 template <int BM, int BN, typename InT, int S, int T, bool FAST>
 __device__ __forceinline__ void tile(const op_view &op, int m0, int n0, uint16_t *smem) {
-  if constexpr (FAST && S == 1 && T == 1 && sizeof(InT) == 2 && BM * BN <= 128 * 128) {
-    tile_bf16<BM, BN>(op, m0, n0, smem);
+  if constexpr (FAST && ((sizeof(InT) == 2 && S == 1 && T == 1) || (sizeof(InT) == 4 && S >= 2 && BM * BN <= 256 * 128))) {
+    tile_fast<BM, BN, InT, S, T>(op, m0, n0, smem);
     return;
   }
   constexpr int BK = stage_k<BM, BN, S>::value;

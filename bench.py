@@ -225,27 +225,49 @@ def suite(torch, ctx, steps):
         out.append(d)
         print(json.dumps(d), flush=True)
 
+    def tm_graph(calls, reps=steps):
+        """The calls captured once into a graph and replayed: per-call time of the kernels
+        themselves, without the host's launch gaps."""
+        for c in calls:
+            c(sh)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            h = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+            for c in calls:
+                c(h)
+        torch.cuda.synchronize()
+        ms = time_passes(torch, g.replay, reps, 2, s) / len(calls)
+        del g
+        return ms
+
+    # HBM-bound ud functions at the 2048 x 2048 block size. Each is cycled over enough
+    # distinct blocks (> 256 MB) that its reads come from HBM, not the last-level cache.
     n = 2048
     mn = n * n
-    blocks = [ctx.dense(n, n) for _ in range(9)]
-    for q, b in enumerate(blocks):
+    nb = lambda k: [ctx.dense(n, n) for _ in range(k)]
+    pool = nb(24)
+    for q, b in enumerate(pool):
         call("bbts_hip_gen_normal", ctx.h, b.data_ptr, mn, 100 + q, 0, 0.0, 1.0, None)
-    outb = ctx.dense(n, n)
-    ms = tm(lambda: call("bbts_hip_dense_add", ctx.h, blocks[0].ptr, blocks[1].ptr, outb.ptr, sh))
-    emit("dense_add 2048x2048", 12.0 * mn, "GB/s", ms, HBM_PEAK_GBS)
-    arr = (ctypes.c_void_p * 8)(*[b.ptr for b in blocks[:8]])
-    ms = tm(lambda: call("bbts_hip_dense_reduce", ctx.h, arr, 8, outb.ptr, sh))
-    emit("dense_reduce k=8 2048x2048 (fused left fold)", 9 * 4.0 * mn, "GB/s", ms, HBM_PEAK_GBS,
+    outs = nb(8)
+    calls = [lambda h, i=i: call("bbts_hip_dense_add", ctx.h, pool[2 * i].ptr, pool[2 * i + 1].ptr, outs[i].ptr, h)
+             for i in range(8)]
+    emit("dense_add 2048x2048", 12.0 * mn, "GB/s", tm_graph(calls), HBM_PEAK_GBS)
+    arrs = [(ctypes.c_void_p * 8)(*[b.ptr for b in pool[8 * i:8 * i + 8]]) for i in range(3)]
+    calls = [lambda h, i=i: call("bbts_hip_dense_reduce", ctx.h, arrs[i], 8, outs[i].ptr, h) for i in range(3)]
+    emit("dense_reduce k=8 2048x2048 (fused left fold)", 9 * 4.0 * mn, "GB/s", tm_graph(calls), HBM_PEAK_GBS,
          {"pairwise_equivalent_bytes": 3 * 7 * 4 * mn})
-    ms = tm(lambda: call("bbts_hip_dense_relu", ctx.h, blocks[0].ptr, outb.ptr, sh))
-    emit("dense_relu 2048x2048", 8.0 * mn, "GB/s", ms, HBM_PEAK_GBS)
-    dsts = [ctx.dense(n, n) for _ in range(32)]
+    calls = [lambda h, i=i: call("bbts_hip_dense_relu", ctx.h, pool[i].ptr, outs[i].ptr, h) for i in range(8)]
+    emit("dense_relu 2048x2048", 8.0 * mn, "GB/s", tm_graph(calls), HBM_PEAK_GBS)
+    dsts = nb(32)
     darr = (ctypes.c_void_p * 32)(*[d.ptr for d in dsts])
     tb = 260 + 4 * mn
-    ms = tm(lambda: call("bbts_hip_broadcast", ctx.h, blocks[0].ptr, darr, 32, sh))
-    emit("broadcast 2048x2048 block -> 32", 33.0 * tb, "GB/s", ms, HBM_PEAK_GBS)
-    for d in dsts:
+    calls = [lambda h, i=i: call("bbts_hip_broadcast", ctx.h, pool[i].ptr, darr, 32, h) for i in range(2)]
+    emit("broadcast 2048x2048 block -> 32", 33.0 * tb, "GB/s", tm_graph(calls), HBM_PEAK_GBS)
+    for d in dsts + outs + pool[8:]:
         d.free()
+    blocks = pool[:8]
+    outb = ctx.dense(n, n)
 
     # block GEMMs: FP32 drop-in on the C1 / C2 block shapes, split-BF16 forms on FP32 N(0,1)
     out2 = torch.zeros(2, dtype=torch.float64, device="cuda")
@@ -255,8 +277,8 @@ def suite(torch, ctx, steps):
         ctx.gen_normal(bb, 8)
         c = torch.empty(sz * sz, dtype=torch.float32, device="cuda")
         for prec in precs:
-            ms = tm(lambda: call("bbts_hip_gemm", ctx.h, sz, sz, sz, a.data_ptr(), sz, 0, bb.data_ptr(), sz, 0,
-                                 c.data_ptr(), sz, 1.0, prec, sh))
+            ms = tm_graph([lambda h: call("bbts_hip_gemm", ctx.h, sz, sz, sz, a.data_ptr(), sz, 0, bb.data_ptr(), sz,
+                                          0, c.data_ptr(), sz, 1.0, prec, h)])
             ctx.residual(sz, sz, sz, a, sz, 0, bb, sz, 0, c, sz, out2)
             torch.cuda.synchronize()
             v = out2.cpu().numpy()
