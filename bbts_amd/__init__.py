@@ -83,6 +83,8 @@ _SIGS = {
     "bbts_hip_gen_uniform": [_vp, _vp, _sz, _u64, _u64, _f, _f, _vp],
     "bbts_hip_gen_normal": [_vp, _vp, _sz, _u64, _u64, _d, _d, _vp],
     "bbts_hip_f32_to_bf16": [_vp, _vp, _vp, _sz, _vp],
+    "bbts_hip_quant_fp8": [_vp, _vp, ctypes.c_int, ctypes.c_int, _i64, ctypes.c_int, ctypes.c_int, _vp, _i64, _i64,
+                           _vp, _vp],
     "bbts_hip_residual_f64": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _i64, ctypes.c_int, _vp, _i64,
                               ctypes.c_int, _vp, _i64, _vp, _vp],
     "bbts_hip_batch_create": [_vp, _pvp],

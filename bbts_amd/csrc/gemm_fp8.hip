@@ -1,11 +1,466 @@
-// FP8 e4m3 block GEMM with per-block power-of-two scales (placeholder until
-// the gfx950 f8f6f4 MFMA path lands): reports BBTS_E_UNSUPPORTED.
-#include "common.h"
+// FP8 e4m3 block GEMM with a power-of-two scale per block, FP32 accumulation, and an
+// optional split-precision refinement step.
+//
+// Each FP32 operand block X gets one scale 2^e, e = ceil(log2(amax|X| / 448)), so X / 2^e
+// fits the e4m3 range (OCP E4M3FN, max normal 448) and the division is exact.  The hi
+// plane is X_hi = e4m3(X / 2^e1) (round to nearest even).  With refinement the residual
+// R = X - 2^e1 X_hi (exact in FP32) gets its own scale and plane X_lo = e4m3(R / 2^e2), and
+// the product keeps the three leading terms A_hi B_hi + A_hi B_lo + A_lo B_hi; without it,
+// only A_hi B_hi.
+//
+// Quantisation writes both operands k-contiguous (A as M x K, B transposed to N x K), rows
+// padded with zeros to a multiple of 16 bytes, so every GEMM load is a whole 16-byte vector
+// and any transpose flag of the caller is absorbed by the quantisation pass.
+//
+// The GEMM tile is the same pipeline as the BF16 tile: 256 lanes as 2 x 2 wave64 over a
+// 128 x 128 output tile, operands read through buffer descriptors (out-of-range lanes read
+// zeros), two register sets keep global reads two stages ahead of the MFMAs, two LDS
+// buffers with XOR-swizzled 16-byte chunks, one barrier per stage.  MFMA:
+// v_mfma_scale_f32_32x32x64_f8f6f4 with e4m3 operands (lane l supplies row l % 32 and the
+// 32 bytes k = 32 (l / 32) + [0, 32) of each 64-deep step; checked on the device), the
+// planes' scale exponents passed as the instruction's E8M0 block scales, so all products
+// accumulate into one FP32 accumulator.
+#include "gemm_bf16.cuh"
 
 namespace bbts_hip {
-int launch_gemm_fp8(bbts_hip_ctx *ctx, const gemm_args &g, int refine, hipStream_t s) {
-  (void)ctx; (void)g; (void)refine; (void)s;
-  set_error("FP8 GEMM is not built yet");
-  return BBTS_E_UNSUPPORTED;
+namespace {
+
+using namespace bf16gemm;
+
+typedef int v8i __attribute__((ext_vector_type(8)));
+
+// float -> OCP e4m3 code, round to nearest even, saturating at +-448 (|v| <= 448 here)
+__device__ __forceinline__ uint32_t e4m3_rne(float v) {
+  const uint32_t bits = __float_as_uint(v);
+  const uint32_t sign = (bits >> 24) & 0x80u;
+  uint32_t u = bits & 0x7fffffffu;
+  uint32_t code;
+  if (u < (121u << 23)) {  // |v| < 2^-6: subnormal codes m * 2^-9, m = 0..8 (8 is the min normal)
+    code = (uint32_t)rintf(__uint_as_float(u) * 512.0f);
+  } else {
+    u += 0x7ffffu + ((u >> 20) & 1u);  // round the 23-bit mantissa to 3 bits, ties to even
+    code = (u >> 20) - (120u << 3);     // rebias the exponent from 127 to 7
+    if (code > 0x7eu) code = 0x7eu;
+  }
+  return sign | code;
 }
+
+__device__ __forceinline__ float e4m3_dec(uint32_t q) {
+  const uint32_t e = (q >> 3) & 15u, m = q & 7u;
+  const float mag = e == 0 ? (float)m * (1.0f / 512.0f) : __uint_as_float(((e + 120u) << 23) | (m << 20));
+  return (q & 0x80u) ? -mag : mag;
+}
+
+// e = ceil(log2(amax / 448)); 0 for an all-zero block
+__device__ __forceinline__ int scale_exp(uint32_t amax_bits) {
+  const float a = __uint_as_float(amax_bits);
+  if (!(a > 0.0f)) return 0;
+  int x;
+  const float f = frexpf(a / 448.0f, &x);  // a/448 = f 2^x, f in [0.5, 1)
+  return f == 0.5f ? x - 1 : x;
+}
+
+__device__ __forceinline__ float block_max(float v, float *red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  float r = red[0];
+  for (int i = 1; i < (int)(blockDim.x >> 6); ++i) r = fmaxf(r, red[i]);
+  return r;
+}
+
+__global__ void zero_words_kernel(uint32_t *p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+// amax |X| over batch x rows x cols (row stride ld, batch stride bs), into *out (float bits)
+__global__ __launch_bounds__(256) void amax_kernel(const float *__restrict__ X, int rows, int cols, int64_t ld,
+                                                   int64_t bs, uint32_t *out) {
+  __shared__ float red[4];
+  const float *x = X + (int64_t)blockIdx.z * bs;
+  float m = 0.0f;
+  for (int r = blockIdx.y; r < rows; r += gridDim.y)
+    for (int c = blockIdx.x * 256 + threadIdx.x; c < cols; c += gridDim.x * 256) m = fmaxf(m, fabsf(x[(int64_t)r * ld + c]));
+  m = block_max(m, red);
+  if (threadIdx.x == 0 && m > 0.0f) atomicMax(out, __float_as_uint(m));
+}
+
+// One 64 x 64 source tile -> e4m3 codes in destination orientation (TRANS: dest row = source
+// column).  LO = false: the hi plane (and, with want_lo_amax, the residual's amax); LO = true:
+// the lo plane of the residual.  sc[0] / sc[1]: amax bits of X and of the residual;
+// se[0] / se[1]: their scale exponents (written by block 0).
+template <bool TRANS, bool LO>
+__global__ __launch_bounds__(256) void quant_kernel(const float *__restrict__ X, int rows, int cols, int64_t ld,
+                                                    int64_t bs, uint8_t *__restrict__ Q, int64_t ldq, int64_t qbs,
+                                                    uint32_t *sc, int *se, int want_lo_amax) {
+  __shared__ uint8_t tq[64][68];
+  __shared__ float red[4];
+  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  const float *x = X + (int64_t)blockIdx.z * bs;
+  const int e1 = scale_exp(sc[0]);
+  const int e2 = LO ? scale_exp(sc[1]) : 0;
+  float rmax = 0.0f;
+  for (int i = threadIdx.x; i < 64 * 64; i += 256) {
+    const int sr = i >> 6, scl = i & 63;
+    const int gr = r0 + sr, gc = c0 + scl;
+    const float v = (gr < rows && gc < cols) ? x[(int64_t)gr * ld + gc] : 0.0f;
+    const uint32_t qh = e4m3_rne(ldexpf(v, -e1));
+    uint32_t q = qh;
+    if (LO || want_lo_amax) {
+      const float res = v - ldexpf(e4m3_dec(qh), e1);  // exact
+      if (LO) q = e4m3_rne(ldexpf(res, -e2));
+      else rmax = fmaxf(rmax, fabsf(res));
+    }
+    if (TRANS) tq[scl][sr] = (uint8_t)q;
+    else tq[sr][scl] = (uint8_t)q;
+  }
+  if (!LO && want_lo_amax) {
+    rmax = block_max(rmax, red);
+    if (threadIdx.x == 0 && rmax > 0.0f) atomicMax(&sc[1], __float_as_uint(rmax));
+  }
+  __syncthreads();
+  if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0) se[LO ? 1 : 0] = LO ? e2 : e1;
+  const int dr0 = TRANS ? c0 : r0, dc0 = TRANS ? r0 : c0;
+  const int drows = TRANS ? cols : rows;
+  uint8_t *q = Q + (int64_t)blockIdx.z * qbs;
+  for (int i = threadIdx.x; i < 64 * 16; i += 256) {
+    const int drl = i >> 4, dcl = (i & 15) * 4;
+    const int gdr = dr0 + drl, gdc = dc0 + dcl;
+    if (gdr < drows && gdc < ldq) {
+      const uint32_t w = (uint32_t)tq[drl][dcl] | ((uint32_t)tq[drl][dcl + 1] << 8) |
+                         ((uint32_t)tq[drl][dcl + 2] << 16) | ((uint32_t)tq[drl][dcl + 3] << 24);
+      *reinterpret_cast<uint32_t *>(q + (int64_t)gdr * ldq + gdc) = w;
+    }
+  }
+}
+
+struct fp8_view {
+  const uint8_t *A, *B;     // hi planes; a lo plane follows at + planeA / + planeB bytes
+  int64_t planeA, planeB;   // bytes between planes
+  int64_t lda, ldb;         // bytes per k-contiguous row (multiples of 16, zero padded)
+  int64_t sa, sb, sc;       // batch strides: bytes, bytes, floats
+  float *C;
+  int64_t ldc;
+  int M, N, Kp;             // Kp: K rounded up to 16
+  float alpha;
+  const int *sexp;          // scale exponents: A hi, A lo, B hi, B lo
+  uint32_t *meta_dst;
+  uint32_t w0, w1;
+};
+
+// element index of (row, byte) in a [rows][RB] byte tile with XOR-swizzled 16-byte chunks
+template <int RB>
+__device__ __forceinline__ int swb(int row, int byte) {
+  const int x = RB == 128 ? (row & 7) : ((row >> 1) & 3);
+  return row * RB + ((((byte >> 4) ^ x)) << 4) + (byte & 15);
+}
+
+template <int BM, int BN, int S, int T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void gemm_fp8_kernel(fp8_view v, int tiles_m,
+                                                                                                int tiles_n) {
+  constexpr int BK = S == 1 ? 128 : 64;  // k bytes per stage
+  constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 32, FN = WN / 32;
+  constexpr int VPR = BK / 16, RS = kThreads / VPR;
+  constexpr int A_PER = BM / RS, B_PER = BN / RS;
+  constexpr int AE = BM * BK, BE = BN * BK;
+  extern __shared__ __align__(16) uint8_t smem8[];
+  uint8_t *As = smem8;                 // [2][S][BM][BK]
+  uint8_t *Bs = smem8 + 2 * S * AE;    // [2][S][BN][BK]
+
+  if (v.meta_dst != nullptr && blockIdx.x == 0 && blockIdx.z == 0 && threadIdx.x == 0) {
+    v.meta_dst[0] = v.w0;
+    v.meta_dst[1] = v.w1;
+  }
+  int tm, tn;
+  {
+    // tiles in groups of 8 rows, column by column inside a group
+    const int t = blockIdx.x, width = 8 * tiles_n, first = (t / width) * 8;
+    const int gs = tiles_m - first < 8 ? tiles_m - first : 8, in = t % width;
+    tm = first + in % gs;
+    tn = in / gs;
+  }
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int M = v.M, N = v.N, Kp = v.Kp;
+  const uint8_t *Ab = v.A + (int64_t)blockIdx.z * v.sa;
+  const uint8_t *Bb = v.B + (int64_t)blockIdx.z * v.sb;
+  float *C = v.C + (int64_t)blockIdx.z * v.sc;
+  const __amdgpu_buffer_rsrc_t rA = make_rsrc(Ab, (S - 1) * v.planeA + (int64_t)(M - 1) * v.lda + Kp);
+  const __amdgpu_buffer_rsrc_t rB = make_rsrc(Bb, (S - 1) * v.planeB + (int64_t)(N - 1) * v.ldb + Kp);
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int vc = tid % VPR, vrow = tid / VPR;
+  uint32_t va[A_PER], vb[B_PER];
+#pragma unroll
+  for (int r = 0; r < A_PER; ++r) {
+    const int gm = m0 + vrow + RS * r;
+    va[r] = gm < M ? (uint32_t)((int64_t)gm * v.lda + vc * 16) : kOOB;
+  }
+#pragma unroll
+  for (int r = 0; r < B_PER; ++r) {
+    const int gn = n0 + vrow + RS * r;
+    vb[r] = gn < N ? (uint32_t)((int64_t)gn * v.ldb + vc * 16) : kOOB;
+  }
+
+  auto load = [&](uint4 (&ra)[S][A_PER], uint4 (&rb)[S][B_PER], int k0) {
+    const bool ok = vc * 16 < Kp - k0;
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+#pragma unroll
+      for (int r = 0; r < A_PER; ++r)
+        ra[q][r] = bload16(rA, ok ? va[r] : kOOB, (uint32_t)(q * v.planeA + k0));
+#pragma unroll
+      for (int r = 0; r < B_PER; ++r)
+        rb[q][r] = bload16(rB, ok ? vb[r] : kOOB, (uint32_t)(q * v.planeB + k0));
+    }
+  };
+  auto store = [&](const uint4 (&ra)[S][A_PER], const uint4 (&rb)[S][B_PER], int buf) {
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      uint8_t *as = As + (buf * S + q) * AE;
+      uint8_t *bs = Bs + (buf * S + q) * BE;
+#pragma unroll
+      for (int r = 0; r < A_PER; ++r) *reinterpret_cast<uint4 *>(&as[swb<BK>(vrow + RS * r, vc * 16)]) = ra[q][r];
+#pragma unroll
+      for (int r = 0; r < B_PER; ++r) *reinterpret_cast<uint4 *>(&bs[swb<BK>(vrow + RS * r, vc * 16)]) = rb[q][r];
+    }
+  };
+
+  int codeA[S], codeB[S];
+#pragma unroll
+  for (int q = 0; q < S; ++q) {
+    codeA[q] = v.sexp[q] + 127;  // E8M0 block scale 2^e
+    codeB[q] = v.sexp[2 + q] + 127;
+  }
+
+  f16v acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
+
+  const int half = lane >> 5, col = lane & 31;
+  auto frag = [&](const uint8_t *t, int row, int kk) {
+    const int b0 = kk * 64 + half * 32;
+    const uint4 lo = *reinterpret_cast<const uint4 *>(&t[swb<BK>(row, b0)]);
+    const uint4 hi = *reinterpret_cast<const uint4 *>(&t[swb<BK>(row, b0 + 16)]);
+    struct {
+      uint4 a, b;
+    } both{lo, hi};
+    return __builtin_bit_cast(v8i, both);
+  };
+  auto compute = [&](int buf) {
+#pragma unroll
+    for (int kk = 0; kk < BK / 64; ++kk) {
+      v8i af[S][FM], bfr[S][FN];
+#pragma unroll
+      for (int q = 0; q < S; ++q) {
+        const uint8_t *as = As + (buf * S + q) * AE;
+        const uint8_t *bs = Bs + (buf * S + q) * BE;
+#pragma unroll
+        for (int i = 0; i < FM; ++i) af[q][i] = frag(as, wm * WM + i * 32 + col, kk);
+#pragma unroll
+        for (int j = 0; j < FN; ++j) bfr[q][j] = frag(bs, wn * WN + j * 32 + col, kk);
+      }
+      // small terms first, the leading product last
+#pragma unroll
+      for (int d = T - 1; d >= 0; --d)
+#pragma unroll
+        for (int qa = 0; qa < S; ++qa) {
+          const int qb = d - qa;
+          if (qb < 0 || qb >= S) continue;
+#pragma unroll
+          for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+              acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(af[qa][i], bfr[qb][j], acc[i][j], 0, 0, 0,
+                                                                         codeA[qa], 0, codeB[qb]);
+        }
+    }
+  };
+
+  uint4 a0[S][A_PER], b0[S][B_PER], a1[S][A_PER], b1[S][B_PER];
+  const int nk = (Kp + BK - 1) / BK;
+  load(a0, b0, 0);
+  load(a1, b1, BK);
+  store(a0, b0, 0);
+  __syncthreads();
+  int kt = 0;
+  for (; kt + 1 < nk; kt += 2) {
+    load(a0, b0, (kt + 2) * BK);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(0);
+    store(a1, b1, 1);
+    __syncthreads();
+    load(a1, b1, (kt + 3) * BK);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(1);
+    store(a0, b0, 0);
+    __syncthreads();
+  }
+  if (kt < nk) compute(0);
+
+  const __amdgpu_buffer_rsrc_t rC = make_rsrc(C, ((int64_t)(M - 1) * v.ldc + N) * 4);
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int gn = n0 + wn * WN + j * 32 + col;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int gm = m0 + wm * WM + i * 32 + (e >> 2) * 8 + half * 4 + (e & 3);
+        const uint32_t off = (gm < M && gn < N) ? (uint32_t)(((int64_t)gm * v.ldc + gn) * 4) : kOOB;
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v.alpha * acc[i][j][e]), rC, off, 0u, 0);
+      }
+    }
+}
+
+template <int S>
+int gemm_fp8_smem() {
+  return 2 * S * (128 + 128) * (S == 1 ? 128 : 64);
+}
+
+int quantize(const float *X, int rows, int cols, int64_t ld, int64_t bs, int batch, bool trans, int S,
+             uint8_t *Q, int64_t ldq, int64_t qbs, int64_t plane, uint32_t *sc, int *se, hipStream_t s) {
+  {
+    const int gx = (cols + 255) / 256 < 64 ? (cols + 255) / 256 : 64;
+    const int gy = rows < 1024 ? rows : 1024;
+    hipLaunchKernelGGL(amax_kernel, dim3((unsigned)gx, (unsigned)gy, (unsigned)batch), dim3(256), 0, s, X, rows, cols,
+                       ld, bs, sc);
+    BBTS_LAUNCH_CHECK();
+  }
+  const dim3 grid((unsigned)((cols + 63) / 64), (unsigned)((rows + 63) / 64), (unsigned)batch);
+  auto hi = trans ? quant_kernel<true, false> : quant_kernel<false, false>;
+  hipLaunchKernelGGL(hi, grid, dim3(256), 0, s, X, rows, cols, ld, bs, Q, ldq, qbs, sc, se, S > 1 ? 1 : 0);
+  BBTS_LAUNCH_CHECK();
+  if (S > 1) {
+    auto lo = trans ? quant_kernel<true, true> : quant_kernel<false, true>;
+    hipLaunchKernelGGL(lo, grid, dim3(256), 0, s, X, rows, cols, ld, bs, Q + plane, ldq, qbs, sc, se, 0);
+    BBTS_LAUNCH_CHECK();
+  }
+  return BBTS_OK;
+}
+
+}  // namespace
+
+// FP32 operands of gemm_args -> C = alpha * op(A) op(B) through FP8 e4m3 planes.
+int launch_gemm_fp8(bbts_hip_ctx *ctx, const gemm_args &g, int refine, hipStream_t s) {
+  BBTS_REQUIRE(g.M >= 0 && g.N >= 0 && g.K >= 0 && g.batch >= 1, BBTS_E_INVALID, "bad GEMM sizes");
+  if (g.M == 0 || g.N == 0) return BBTS_OK;
+  const int S = refine ? 2 : 1;
+  const int64_t Kp = ((int64_t)g.K + 15) / 16 * 16;
+  const int64_t Kq = Kp > 0 ? Kp : 16;
+  const int64_t planeA = (int64_t)g.batch * g.M * Kq, planeB = (int64_t)g.batch * g.N * Kq;
+  BBTS_REQUIRE(S * planeA < (int64_t(1) << 31) && S * planeB < (int64_t(1) << 31) &&
+                   (int64_t)g.batch * g.M * g.ldc * 4 < (int64_t(1) << 31),
+               BBTS_E_UNSUPPORTED, "FP8 operands beyond 2 GiB");
+  const size_t head = 256;
+  const size_t bytes = head + (size_t)(S * planeA) + 256 + (size_t)(S * planeB);
+  void *tmp = nullptr;
+  size_t rounded = 0;
+  int rc = pool_alloc(ctx, bytes, &tmp, &rounded);
+  if (rc) return rc;
+  uint32_t *sc = reinterpret_cast<uint32_t *>(tmp);  // [0,1] A amax, [2,3] B amax
+  int *se = reinterpret_cast<int *>(sc + 8);           // [0,1] A exponents, [2,3] B exponents
+  uint8_t *QA = reinterpret_cast<uint8_t *>(tmp) + head;
+  uint8_t *QB = QA + ((S * planeA + 255) / 256) * 256;
+  hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, s, sc, 16);
+  rc = BBTS_OK;
+  if (g.K > 0) {
+    // A: stored M x K (ta = 0) or K x M (ta = 1); B: K x N (tb = 0, transposed by the pass) or N x K
+    if (!rc) rc = quantize(g.A, g.ta ? g.K : g.M, g.ta ? g.M : g.K, g.lda, g.sa, g.batch, g.ta != 0, S, QA, Kq,
+                           (int64_t)g.M * Kq, planeA, sc, se, s);
+    if (!rc) rc = quantize(g.B, g.tb ? g.N : g.K, g.tb ? g.K : g.N, g.ldb, g.sb, g.batch, g.tb == 0, S, QB, Kq,
+                           (int64_t)g.N * Kq, planeB, sc + 2, se + 2, s);
+  }
+  if (!rc) {
+    fp8_view v{};
+    v.A = QA;
+    v.B = QB;
+    v.planeA = planeA;
+    v.planeB = planeB;
+    v.lda = Kq;
+    v.ldb = Kq;
+    v.sa = (int64_t)g.M * Kq;
+    v.sb = (int64_t)g.N * Kq;
+    v.sc = g.sc;
+    v.C = g.C;
+    v.ldc = g.ldc;
+    v.M = g.M;
+    v.N = g.N;
+    v.Kp = (int)Kp;
+    v.alpha = g.alpha;
+    v.sexp = se;
+    v.meta_dst = g.meta_dst;
+    v.w0 = g.w0;
+    v.w1 = g.w1;
+    const int tiles_m = (g.M + 127) / 128, tiles_n = (g.N + 127) / 128;
+    auto fn = S == 1 ? gemm_fp8_kernel<128, 128, 1, 1> : gemm_fp8_kernel<128, 128, 2, 2>;
+    const int smem = S == 1 ? gemm_fp8_smem<1>() : gemm_fp8_smem<2>();
+    hipError_t e = hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, smem);
+    if (e != hipSuccess) {
+      set_error("hipFuncSetAttribute failed: %s", hipGetErrorString(e));
+      rc = BBTS_E_HIP;
+    } else {
+      hipLaunchKernelGGL(fn, dim3((unsigned)(tiles_m * tiles_n), 1, (unsigned)g.batch), dim3(256), (unsigned)smem, s, v,
+                         tiles_m, tiles_n);
+      hipError_t le = hipGetLastError();
+      if (le != hipSuccess) {
+        set_error("FP8 GEMM launch failed: %s", hipGetErrorString(le));
+        rc = BBTS_E_HIP;
+      }
+    }
+  }
+  // the quantised planes are released once the stream has consumed them
+  hipError_t se2 = hipStreamSynchronize(s);
+  pool_release(ctx, tmp, rounded);
+  if (!rc && se2 != hipSuccess) {
+    set_error("hipStreamSynchronize failed: %s", hipGetErrorString(se2));
+    rc = BBTS_E_HIP;
+  }
+  return rc;
+}
+
 }  // namespace bbts_hip
+
+using namespace bbts_hip;
+
+extern "C" int bbts_hip_quant_fp8(bbts_hip_ctx *ctx, const float *X, int rows, int cols, int64_t ld, int trans,
+                                  int planes, uint8_t *Q, int64_t ldq, int64_t plane_bytes, int *exps,
+                                  bbts_stream_t stream) {
+  int rc = bind(ctx);
+  if (rc) return rc;
+  BBTS_REQUIRE(rows >= 0 && cols >= 0 && (planes == 1 || planes == 2), BBTS_E_INVALID, "bad quantisation arguments");
+  const int64_t len = trans ? rows : cols;
+  BBTS_REQUIRE(ldq >= (len + 15) / 16 * 16 && ldq % 16 == 0, BBTS_E_INVALID, "ldq must be >= row length rounded to 16");
+  BBTS_REQUIRE(planes == 1 || plane_bytes >= (int64_t)(trans ? cols : rows) * ldq, BBTS_E_INVALID,
+               "planes overlap");
+  BBTS_REQUIRE(reinterpret_cast<uintptr_t>(Q) % 4 == 0, BBTS_E_INVALID, "Q must be 4-byte aligned");
+  if (rows == 0 || cols == 0) return BBTS_OK;
+  hipStream_t s = (hipStream_t)stream;
+  void *tmp = nullptr;
+  size_t rounded = 0;
+  if ((rc = pool_alloc(ctx, 256, &tmp, &rounded))) return rc;
+  uint32_t *sc = reinterpret_cast<uint32_t *>(tmp);
+  int *se = reinterpret_cast<int *>(sc + 8);
+  hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, s, sc, 16);
+  rc = quantize(X, rows, cols, ld, 0, 1, trans != 0, planes, Q, ldq, 0, plane_bytes, sc, se, s);
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(exps, se, sizeof(int) * (size_t)planes, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) {
+      set_error("hipMemcpyAsync failed: %s", hipGetErrorString(e));
+      rc = BBTS_E_HIP;
+    }
+  }
+  hipError_t e2 = hipStreamSynchronize(s);
+  pool_release(ctx, tmp, rounded);
+  if (!rc && e2 != hipSuccess) {
+    set_error("hipStreamSynchronize failed: %s", hipGetErrorString(e2));
+    rc = BBTS_E_HIP;
+  }
+  return rc;
+}

@@ -141,6 +141,15 @@ int bbts_hip_gen_normal(bbts_hip_ctx *ctx, float *out, size_t n, uint64_t seed, 
                         double stdev, bbts_stream_t stream);
 /* Converts FP32 to BF16 (round to nearest even). */
 int bbts_hip_f32_to_bf16(bbts_hip_ctx *ctx, const float *in, uint16_t *out, size_t n, bbts_stream_t stream);
+/* FP8 e4m3 quantisation of an FP32 rows x cols matrix (row stride ld) as used by the
+ * BBTS_PREC_FP8X* GEMMs: one power-of-two scale 2^e per block, e = ceil(log2(amax / 448)),
+ * codes = e4m3(x / 2^e) (OCP E4M3FN, round to nearest even).  planes = 2 adds the residual's
+ * plane (its own scale) at Q + plane_bytes.  trans = 1 writes the transpose (cols x rows).
+ * Output rows are ldq bytes (a multiple of 16, >= the row length rounded up to 16, zero
+ * padded).  exps: device int[planes], receives the exponents.  No counterpart in the
+ * reference (its matmul is FP32 only); exposed so the quantised bytes can be checked. */
+int bbts_hip_quant_fp8(bbts_hip_ctx *ctx, const float *X, int rows, int cols, int64_t ld, int trans, int planes,
+                       uint8_t *Q, int64_t ldq, int64_t plane_bytes, int *exps, bbts_stream_t stream);
 /* FP64 residual check: out2[0] = sum (C - A·B)^2, out2[1] = sum (A·B)^2 over the M x N product,
  * with A·B formed in FP64 on the GPU (row-major operands; a_bf16/b_bf16 select BF16 operand storage).
  * out2 is a device pointer to 2 doubles, overwritten. */

@@ -589,3 +589,95 @@ void orc_permute_raw(const int64_t *dims, const int32_t *ordering, int rank, flo
   for (int r = 0; r < rank; ++r) ord[r] = ordering[r];
   orc_permute(dims, ord, rank, buf, tmp);
 }
+
+/* ---- FP8 e4m3 precision forms ---------------------------------------------
+ * Not a restatement of the reference, whose matmul is FP32 only
+ * (dense_matrix_mult.cc:82): the definition this build's FP8 GEMMs follow, checked
+ * against brute force (every code, nearest-even ties) in tests/test_oracle.py.
+ * OCP E4M3FN: 1 sign, 4 exponent (bias 7), 3 mantissa bits; subnormals m * 2^-9;
+ * largest finite 448; code 0x7f is NaN (never produced here). */
+float orc_e4m3_decode(uint8_t q) {
+  int e = (q >> 3) & 15, m = q & 7;
+  float mag = e == 0 ? (float)m / 512.0f : ldexpf(1.0f + (float)m / 8.0f, e - 7);
+  return (q & 0x80) ? -mag : mag;
+}
+
+/* nearest code, ties to the even code, saturating at +-448 */
+uint8_t orc_e4m3_encode(float v) {
+  uint32_t bits;
+  memcpy(&bits, &v, 4);
+  uint8_t sign = (uint8_t)((bits >> 24) & 0x80u);
+  float a = fabsf(v);
+  if (a >= 448.0f) return (uint8_t)(sign | 0x7e);
+  if (a < 0.015625f) return (uint8_t)(sign | (uint8_t)nearbyintf(a * 512.0f)); /* below 2^-6 */
+  int ex;
+  frexpf(a, &ex);                     /* a = f 2^ex, f in [0.5, 1): exponent of a is ex - 1 */
+  int e = ex - 1;                     /* 2^e <= a < 2^(e+1), e >= -6 */
+  float m = nearbyintf(ldexpf(a, 3 - e)) - 8.0f; /* (a / 2^e - 1) * 8, rounded */
+  if (m >= 8.0f) { m = 0.0f; e += 1; }
+  int code = ((e + 7) << 3) | (int)m;
+  if (code > 0x7e) code = 0x7e;
+  return (uint8_t)(sign | code);
+}
+
+/* e = ceil(log2(amax / 448)); 0 for an all-zero block */
+int orc_fp8_scale_exp(float amax) {
+  if (!(amax > 0.0f)) return 0;
+  int x;
+  float f = frexpf(amax / 448.0f, &x);
+  return f == 0.5f ? x - 1 : x;
+}
+
+/* Quantise rows x cols (row stride ld) into planes (hi, and with planes = 2 the residual's
+ * lo plane at Q + plane), rows of ldq bytes, transposed when trans; exps[planes]. */
+void orc_quant_fp8(const float *X, int rows, int cols, int64_t ld, int trans, int planes, uint8_t *Q, int64_t ldq,
+                   int64_t plane, int *exps) {
+  float amax = 0.0f, rmax = 0.0f;
+  for (int r = 0; r < rows; ++r)
+    for (int c = 0; c < cols; ++c) amax = fmaxf(amax, fabsf(X[(int64_t)r * ld + c]));
+  int e1 = orc_fp8_scale_exp(amax);
+  int drows = trans ? cols : rows;
+  memset(Q, 0, (size_t)(drows * ldq));
+  if (planes > 1) memset(Q + plane, 0, (size_t)(drows * ldq));
+  for (int r = 0; r < rows; ++r)
+    for (int c = 0; c < cols; ++c) {
+      float v = X[(int64_t)r * ld + c];
+      uint8_t q = orc_e4m3_encode(ldexpf(v, -e1));
+      int64_t d = trans ? (int64_t)c * ldq + r : (int64_t)r * ldq + c;
+      Q[d] = q;
+      float res = v - ldexpf(orc_e4m3_decode(q), e1);
+      rmax = fmaxf(rmax, fabsf(res));
+    }
+  exps[0] = e1;
+  if (planes > 1) {
+    int e2 = orc_fp8_scale_exp(rmax);
+    exps[1] = e2;
+    for (int r = 0; r < rows; ++r)
+      for (int c = 0; c < cols; ++c) {
+        float v = X[(int64_t)r * ld + c];
+        float res = v - ldexpf(orc_e4m3_decode(orc_e4m3_encode(ldexpf(v, -e1))), e1);
+        int64_t d = trans ? (int64_t)c * ldq + r : (int64_t)r * ldq + c;
+        Q[plane + d] = orc_e4m3_encode(ldexpf(res, -e2));
+      }
+  }
+}
+
+/* C (M x N, FP64) = sum over the kept terms (s_a + s_b < terms) of
+ * 2^(eA[s_a] + eB[s_b]) QA_s_a · QB_s_b^T, QA M x K and QB N x K k-contiguous planes. */
+void orc_gemm_fp8(const uint8_t *QA, const uint8_t *QB, int M, int N, int K, int64_t lda, int64_t ldb,
+                  int64_t planeA, int64_t planeB, const int *eA, const int *eB, int planes, int terms, double *C) {
+  for (int64_t i = 0; i < (int64_t)M * N; ++i) C[i] = 0.0;
+  for (int sa = 0; sa < planes; ++sa)
+    for (int sb = 0; sb < planes; ++sb) {
+      if (sa + sb >= terms) continue;
+      double sc = ldexp(1.0, eA[sa] + eB[sb]);
+      for (int m = 0; m < M; ++m)
+        for (int n = 0; n < N; ++n) {
+          const uint8_t *a = QA + sa * planeA + (int64_t)m * lda;
+          const uint8_t *b = QB + sb * planeB + (int64_t)n * ldb;
+          double acc = 0.0;
+          for (int k = 0; k < K; ++k) acc += (double)orc_e4m3_decode(a[k]) * (double)orc_e4m3_decode(b[k]);
+          C[(int64_t)m * N + n] += sc * acc;
+        }
+    }
+}

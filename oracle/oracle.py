@@ -74,6 +74,16 @@ def lib():
         L.orc_gemm_f64.argtypes = [f32p, f32p, f64p, ctypes.c_uint32, ctypes.c_uint32,
                                    ctypes.c_uint32, ctypes.c_int, ctypes.c_int]
         L.orc_dgemm_rm.argtypes = [f64p, f64p, f64p] + [ctypes.c_uint32] * 6
+        L.orc_e4m3_decode.restype = ctypes.c_float
+        L.orc_e4m3_decode.argtypes = [ctypes.c_uint8]
+        L.orc_e4m3_encode.restype = ctypes.c_uint8
+        L.orc_e4m3_encode.argtypes = [ctypes.c_float]
+        L.orc_fp8_scale_exp.restype = ctypes.c_int
+        L.orc_fp8_scale_exp.argtypes = [ctypes.c_float]
+        L.orc_quant_fp8.argtypes = [f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                    u8p, ctypes.c_int64, ctypes.c_int64, i32p]
+        L.orc_gemm_fp8.argtypes = [u8p, u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
+                                   ctypes.c_int64, ctypes.c_int64, i32p, i32p, ctypes.c_int, ctypes.c_int, f64p]
         L.orc_sgemm_provider.restype = ctypes.c_char_p
         L.orc_sgemm_threads.restype = ctypes.c_int
         L.orc_sgemm_threads.argtypes = [ctypes.c_int]
@@ -275,3 +285,44 @@ def permute(buf, dims, ordering):
     o = np.array(list(ordering) + [0] * (4 - len(ordering)), dtype=np.int32)
     lib().orc_permute_raw(_p(d, i64p), _p(o, i32p), len(dims), _p(b, f32p), _p(tmp, f32p))
     return b
+
+
+# ---- FP8 e4m3 forms (this build's definition; the reference's matmul is FP32 only)
+def e4m3_decode(q):
+    return lib().orc_e4m3_decode(int(q))
+
+
+def e4m3_encode(v):
+    return lib().orc_e4m3_encode(float(v))
+
+
+def fp8_scale_exp(amax):
+    return lib().orc_fp8_scale_exp(float(amax))
+
+
+def quant_fp8(x, trans=False, planes=1):
+    """(Q, exps): Q is uint8 [planes, drows, ldq] (drows = cols if trans), ldq = row length
+    rounded up to 16, zero padded."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    rows, cols = x.shape
+    drows, dlen = (cols, rows) if trans else (rows, cols)
+    ldq = max(16, (dlen + 15) // 16 * 16)
+    q = np.zeros((planes, drows, ldq), dtype=np.uint8)
+    exps = np.zeros(planes, dtype=np.int32)
+    lib().orc_quant_fp8(_p(x, f32p), rows, cols, cols, int(trans), planes, _p(q, u8p), ldq, drows * ldq,
+                        _p(exps, i32p))
+    return q, exps
+
+
+def gemm_fp8(qa, ea, qb, eb, K, terms):
+    """FP64 sum of the kept plane products 2^(ea[s]+eb[t]) QA_s QB_t^T (s + t < terms)."""
+    qa = np.ascontiguousarray(qa, dtype=np.uint8)
+    qb = np.ascontiguousarray(qb, dtype=np.uint8)
+    planes, M, lda = qa.shape
+    _, N, ldb = qb.shape
+    out = np.empty((M, N), dtype=np.float64)
+    ea = np.ascontiguousarray(ea, dtype=np.int32)
+    eb = np.ascontiguousarray(eb, dtype=np.int32)
+    lib().orc_gemm_fp8(_p(qa, u8p), _p(qb, u8p), M, N, K, lda, ldb, M * lda, N * ldb, _p(ea, i32p), _p(eb, i32p),
+                       planes, terms, _p(out, f64p))
+    return out

@@ -14,6 +14,10 @@
   c and e (512 block contractions of M = N = 1152, K = 2048 through cutensor-format blocks,
   8-way reduces over the c blocks), assembled and checked against the untiled FP64
   contraction on the GPU: <= 5e-6 for FP32 and BF16x6, <= 1e-5 for BF16x3.
+* fnn forward, X 16384 x 8192 . W1 8192^2 -> ReLU -> . W2 8192 x 4096, weights N(0, 1/fan_in),
+  in 2048^2 blocks (192 block products, 48 k = 4 reduces, ReLU on 32 blocks), the output
+  against the FP64 forward: <= 5e-3 with FP8 e4m3 and one refinement step, <= 2e-5 with
+  BF16x3.
 """
 import ctypes
 import math
@@ -190,3 +194,65 @@ def test_contraction_abcd_cdef_tiled(ctx, orc, torch_cuda, prec, tol):
     assert err <= tol
     for blk in list(lhs.values()) + list(rhs.values()) + part + [out]:
         blk.free()
+
+
+@pytest.mark.parametrize("prec,tol", [(6, 5e-3), (2, 2e-5)])
+def test_fnn_forward_2048_blocks(ctx, orc, torch_cuda, prec, tol):
+    """X 16384 x 8192 . W1 8192 x 8192 -> ReLU -> . W2 8192 x 4096, weights N(0, 1/fan_in), in
+    2048^2 blocks: 128 + 64 block products (FP8 e4m3 with one refinement step, or BF16x3), 32 +
+    16 k = 4 reduces and ReLU on the 32 hidden blocks; the output against the FP64 forward."""
+    torch = torch_cuda
+    bs = 2048
+    nb = 4 * bs * bs
+    X = torch.empty((16384, 8192), dtype=torch.float32, device="cuda")
+    W1 = torch.empty((8192, 8192), dtype=torch.float32, device="cuda")
+    W2 = torch.empty((8192, 4096), dtype=torch.float32, device="cuda")
+    ctx.gen_normal(X, seed=61)
+    ctx.gen_normal(W1, seed=62, std=1.0 / math.sqrt(8192))
+    ctx.gen_normal(W2, seed=63, std=1.0 / math.sqrt(8192))
+
+    def blocks(T):
+        out = {}
+        for i in range(T.shape[0] // bs):
+            for j in range(T.shape[1] // bs):
+                b = ctx.dense(bs, bs)
+                src = T[i * bs:(i + 1) * bs, j * bs:(j + 1) * bs].contiguous()
+                _copy(ctx, src.data_ptr(), b.data_ptr, nb)
+                torch.cuda.synchronize()
+                out[i, j] = b
+        return out
+
+    Xb, W1b, W2b = blocks(X), blocks(W1), blocks(W2)
+    print("fnn forward, precision %d: 192 block products" % prec, flush=True)
+
+    def layer(A, B, I, J, K, relu):
+        out = {}
+        for i in range(I):
+            for j in range(J):
+                prods = [ctx.dense(bs, bs) for _ in range(K)]
+                for k in range(K):
+                    ctx.matmul(A[i, k], B[k, j], prods[k], precision=prec)
+                c = ctx.dense(bs, bs)
+                ctx.reduce(prods, c)
+                if relu:
+                    ctx.relu(c, c)
+                ctx.sync()
+                for p in prods:
+                    p.free()
+                out[i, j] = c
+        return out
+
+    H = layer(Xb, W1b, 8, 4, 4, True)
+    Y = layer(H, W2b, 8, 2, 4, False)
+    got = torch.empty((16384, 4096), dtype=torch.float32, device="cuda")
+    tmp = torch.empty((bs, bs), dtype=torch.float32, device="cuda")
+    for (i, j), b in Y.items():
+        _copy(ctx, b.data_ptr, tmp.data_ptr(), nb)
+        torch.cuda.synchronize()
+        got[i * bs:(i + 1) * bs, j * bs:(j + 1) * bs] = tmp
+    ref = torch.relu(X.double() @ W1.double()) @ W2.double()
+    err = (torch.linalg.norm(got.double() - ref) / torch.linalg.norm(ref)).item()
+    print("fnn forward precision %d: relative error %.3g" % (prec, err), flush=True)
+    assert err <= tol
+    for b in list(Xb.values()) + list(W1b.values()) + list(W2b.values()) + list(H.values()) + list(Y.values()):
+        b.free()

@@ -266,3 +266,78 @@ def test_contraction_plan_groups(orc):
     assert info["out_dims"] == (96, 96, 96, 96)
     assert (info["ni"], info["nj"], info["nk"], info["nb"]) == (9216, 16384, 9216, 1)
     assert not (info["t_lhs"] or info["t_rhs"] or info["t_out"])
+
+
+# ---- FP8 e4m3 (the definition the FP8 GEMM forms follow; pinned by brute force) ----
+_FINITE = [c for c in range(256) if (c & 0x7f) != 0x7f]
+
+
+def test_e4m3_decode_table(orc):
+    # OCP E4M3FN: bias 7, subnormals m 2^-9, largest finite 448
+    assert orc.e4m3_decode(0x38) == 1.0 and orc.e4m3_decode(0x7e) == 448.0
+    assert orc.e4m3_decode(0x01) == 2.0 ** -9 and orc.e4m3_decode(0x08) == 2.0 ** -6
+    assert orc.e4m3_decode(0xb8) == -1.0 and orc.e4m3_decode(0x3f) == 1.875
+    vals = sorted({orc.e4m3_decode(c) for c in _FINITE})
+    assert len(vals) == 253  # 126 positive, 126 negative finite values and zero (+0 / -0 coincide)
+
+
+def test_e4m3_encode_roundtrip_every_code(orc):
+    for c in _FINITE:
+        v = orc.e4m3_decode(c)
+        assert orc.e4m3_decode(orc.e4m3_encode(v)) == v
+
+
+def test_e4m3_encode_is_nearest_even(orc):
+    codes = [c for c in _FINITE if c < 0x80]
+    table = np.array([orc.e4m3_decode(c) for c in codes])
+    rng = np.random.default_rng(9)
+    samples = np.concatenate([rng.uniform(0, 448, 3000), np.exp(rng.uniform(np.log(1e-4), np.log(448), 3000)),
+                              (table[:-1] + table[1:]) / 2])  # every midpoint: the ties
+    for v in samples.astype(np.float32):
+        d = np.abs(table - float(v))
+        best = np.flatnonzero(d == d.min())
+        want = codes[best[0]] if len(best) == 1 else [codes[b] for b in best if codes[b] % 2 == 0][0]
+        assert orc.e4m3_encode(float(v)) == want, v
+        assert orc.e4m3_encode(-float(v)) == (want | 0x80)
+
+
+def test_fp8_scale_exponent(orc):
+    rng = np.random.default_rng(10)
+    for a in np.exp(rng.uniform(-20, 20, 500)).astype(np.float32):
+        e = orc.fp8_scale_exp(float(a))
+        assert 2.0 ** e >= a / 448.0 > 2.0 ** (e - 1)
+    assert orc.fp8_scale_exp(448.0) == 0 and orc.fp8_scale_exp(0.0) == 0
+
+
+def test_quant_fp8_planes(orc):
+    x = orc.normal(37 * 50, 11).reshape(37, 50)
+    q, e = orc.quant_fp8(x, trans=False, planes=2)
+    qt, et = orc.quant_fp8(x, trans=True, planes=2)
+    assert q.shape == (2, 37, 64) and qt.shape == (2, 50, 48) and list(e) == list(et)
+    assert np.array_equal(qt[:, :, :37], np.transpose(q[:, :, :50], (0, 2, 1)))
+    assert not q[:, :, 50:].any() and not qt[:, :, 37:].any()  # zero padding
+    dec = np.vectorize(orc.e4m3_decode)
+    hi = dec(q[0, :, :50]) * 2.0 ** e[0]
+    lo = dec(q[1, :, :50]) * 2.0 ** e[1]
+    assert np.abs(hi).max() <= 448 * 2.0 ** e[0] and np.abs(x).max() > 224 * 2.0 ** e[0]
+    res = x.astype(np.float64) - hi
+    # e4m3 keeps 4 significant bits: relative error <= 2^-4 above the subnormal range
+    assert np.all(np.abs(res) <= 2.0 ** -4 * np.abs(x) + 2.0 ** (e[0] - 10))
+    assert np.all(np.abs(res - lo) <= 2.0 ** -4 * np.abs(res) + 2.0 ** (e[1] - 10))
+
+
+def test_gemm_fp8_terms(orc):
+    a = orc.normal(20 * 33, 12).reshape(20, 33)
+    b = orc.normal(33 * 18, 13).reshape(33, 18)
+    qa, ea = orc.quant_fp8(a, planes=2)
+    qb, eb = orc.quant_fp8(b, trans=True, planes=2)
+    dec = np.vectorize(orc.e4m3_decode)
+    A = [dec(qa[s, :, :33]) * 2.0 ** ea[s] for s in range(2)]
+    B = [dec(qb[s, :, :33]).T * 2.0 ** eb[s] for s in range(2)]
+    assert np.allclose(orc.gemm_fp8(qa, ea, qb, eb, 33, 1), A[0] @ B[0], rtol=1e-12, atol=1e-12)
+    want = A[0] @ B[0] + A[0] @ B[1] + A[1] @ B[0]
+    assert np.allclose(orc.gemm_fp8(qa, ea, qb, eb, 33, 2), want, rtol=1e-12, atol=1e-12)
+    ref = a.astype(np.float64) @ b.astype(np.float64)
+    e1 = np.linalg.norm(orc.gemm_fp8(qa, ea, qb, eb, 33, 1) - ref) / np.linalg.norm(ref)
+    e3 = np.linalg.norm(want - ref) / np.linalg.norm(ref)
+    assert e3 < e1 / 4 and e1 < 0.1
