@@ -41,7 +41,7 @@ constexpr int kThreads = 256;
 // 256x256 tile (whose 256 accumulators leave no room for a 64-deep staging set)
 template <int BM, int BN, int S>
 struct stage_k {
-  static constexpr int value = (S == 1 && BM * BN <= 256 * 128) ? 64 : 32;
+  static constexpr int value = (S == 1 && BM * BN <= 128 * 128) ? 64 : 32;
 };
 
 // element index of (row, k) in a [rows][BK] BF16 tile with swizzled 16-byte chunks

@@ -62,9 +62,10 @@ __device__ inline void map_tile2(int lin, int tiles_m, int tiles_n, int *tm, int
 // split-precision tiles keep the allocator's own choice).
 template <int BM, int BN, typename InT, int S>
 struct waves_target {
+  static constexpr int big = (sizeof(InT) == 2 && S == 1) ? 256 * 128 : 128 * 128;  // largest 2-wave tile
   static constexpr int value =
       ((sizeof(InT) == 2 && S == 1) || (sizeof(InT) == 4 && S >= 2))
-          ? (BM * BN >= 256 * 128 ? 1 : (BM * BN >= 128 * 128 ? 2 : (BM * BN >= 128 * 64 ? 3 : 4)))
+          ? (BM * BN > big ? 1 : (BM * BN >= 128 * 128 ? 2 : (BM * BN >= 128 * 64 ? 3 : 4)))
           : 1;
 };
 
@@ -146,7 +147,7 @@ struct cfg_kernels {
 };
 
 int stage_k_of(int cfg, int variant) {
-  return (variant <= 1 && kCfgs[cfg].bm * kCfgs[cfg].bn <= 256 * 128) ? 64 : 32;
+  return (variant <= 1 && kCfgs[cfg].bm * kCfgs[cfg].bn <= 128 * 128) ? 64 : 32;
 }
 
 int smem_bytes(int cfg, int variant) {
@@ -202,7 +203,7 @@ int occupancy(int cfg, int variant) {
   int b = smem_bytes(cfg, variant) + 64;
   int by_lds = (160 * 1024) / b;
   // waves per SIMD the register budget allows (4 waves per workgroup, one per SIMD)
-  static const int kByRegs[] = {1, 2, 3, 4, 3, 1};
+  static const int kByRegs[] = {variant == 0 ? 2 : 1, 2, 3, 4, 3, 1};
   int by_waves = kByRegs[cfg];
   int o = by_lds < by_waves ? by_lds : by_waves;
   return o < 1 ? 1 : o;
