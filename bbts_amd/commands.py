@@ -1,0 +1,156 @@
+"""Writers for the reference's command files, and its bmm stream generator.
+
+`ParsedCommandList` mirrors parsed_command_list_t (src/commands/parsed_command.h:139-231:
+add_apply / add_reduce / add_move / add_delete / serialize), producing the same bytes:
+u64 command count; per command i32 type, ud name (u64 length + bytes), is_gpu (1 byte),
+input and output type lists (u64 count + strings), inputs and outputs (u64 count + the
+libstdc++ layout of std::tuple<tid_t, node_id_t>: i32 node then i32 tid), parameters
+(u64 count + 4-byte words).  `SourceFile` writes compile_source_file_t's text form
+(src/commands/command_compiler.h:391-501, 977-1027).  `generate_bmm` builds the command
+stream of applications/bmm/generate_bmm.cc (uniform blocks, row shuffle, broadcast of B,
+per-block products, add-reduce per output block, deletes).
+
+These produce input for `bbts_amd.runtime.Runtime.load_commands`, which runs a file's
+commands on one GPU.
+"""
+import struct
+
+APPLY, REDUCE, MOVE, DELETE = 0, 1, 2, 3
+
+
+def _word(p):
+    """A command_param_t word: float -> its bits, bool / int -> 32 bits."""
+    if isinstance(p, float):
+        return struct.unpack("<I", struct.pack("<f", p))[0]
+    return int(p) & 0xFFFFFFFF
+
+
+class ParsedCommandList:
+    def __init__(self):
+        self.commands = []
+
+    def add_apply(self, fn, in_types, out_types, is_gpu, inputs, outputs, params=()):
+        self.commands.append((APPLY, fn, bool(is_gpu), list(in_types), list(out_types), list(inputs),
+                              list(outputs), [_word(p) for p in params]))
+
+    def add_reduce(self, fn, in_types, out_types, is_gpu, inputs, output, params=()):
+        self.commands.append((REDUCE, fn, bool(is_gpu), list(in_types), list(out_types), list(inputs), [output],
+                              [_word(p) for p in params]))
+
+    def add_move(self, inp, outputs):
+        self.commands.append((MOVE, "", False, [], [], [inp], list(outputs), []))
+
+    def add_delete(self, inputs):
+        self.commands.append((DELETE, "", False, [], [], list(inputs), [], []))
+
+    def serialize(self):
+        out = bytearray(struct.pack("<Q", len(self.commands)))
+
+        def string(s):
+            b = s.encode()
+            out.extend(struct.pack("<Q", len(b)))
+            out.extend(b)
+
+        for (typ, fn, gpu, tin, tout, ins, outs, params) in self.commands:
+            out.extend(struct.pack("<i", typ))
+            string(fn)
+            out.extend(struct.pack("<B", 1 if gpu else 0))
+            for lst in (tin, tout):
+                out.extend(struct.pack("<Q", len(lst)))
+                for s in lst:
+                    string(s)
+            for lst in (ins, outs):
+                out.extend(struct.pack("<Q", len(lst)))
+                for tid, node in lst:
+                    out.extend(struct.pack("<ii", node, tid))  # libstdc++ tuple: last element first
+            out.extend(struct.pack("<Q", len(params)))
+            for w in params:
+                out.extend(struct.pack("<I", w))
+        return bytes(out)
+
+
+class SourceFile:
+    """compile_source_file_t: function specs (id, name, input types, output types), then commands."""
+
+    def __init__(self):
+        self.functions = []
+        self.commands = []
+
+    def add_function(self, fid, name, in_types, out_types):
+        self.functions.append((fid, name, list(in_types), list(out_types)))
+
+    def add_command(self, fid, kind, inputs, outputs, params=()):
+        self.commands.append((fid, kind, list(inputs), list(outputs), list(params)))
+
+    def text(self):
+        lines = [str(len(self.functions))]
+        for fid, name, tin, tout in self.functions:
+            lines.append(" ".join([str(fid), name, str(len(tin))] + tin + [str(len(tout))] + tout))
+        lines.append(str(len(self.commands)))
+        for fid, kind, ins, outs, params in self.commands:
+            toks = [str(fid), kind, str(len(ins))] + [str(t) for t in ins] + [str(len(outs))]
+            toks += [str(t) for t in outs] + [str(len(params))]
+            for p in params:
+                if isinstance(p, float):
+                    toks += ["float", repr(p)]
+                else:
+                    toks += ["int", str(int(p))]
+            lines.append(" ".join(toks))
+        return "\n".join(lines) + "\n"
+
+
+def generate_bmm(split, num_nodes, block_size, gpu_mult=True, gpu_add=True):
+    """The bmm application's command stream (applications/bmm/generate_bmm.cc): C = A . B with A,
+    B split x split grids of uniform blocks; returns (ParsedCommandList, A tids, B tids, C tids)
+    with the tid grids as {(row, col): tid}."""
+    cmds = ParsedCommandList()
+    tid = 0
+
+    def matrix():
+        nonlocal tid
+        where = {}
+        for r in range(split):
+            for c in range(split):
+                node = (r * split + c) % num_nodes  # std::hash<int> is the identity
+                where[r, c] = (node, tid)
+                cmds.add_apply("uniform", [], ["dense"], False, [], [(tid, node)],
+                               [block_size, block_size, 0.0, 1.0])
+                tid += 1
+        return where
+
+    a, b = matrix(), matrix()
+    to_del = [[] for _ in range(num_nodes)]
+    # A's blocks go to node row % num_nodes
+    for (r, c), (node, t) in sorted(a.items()):
+        target = r % num_nodes
+        if target != node:
+            cmds.add_move((t, node), [(t, target)])
+            to_del[target].append(t)
+    # every B block to every node
+    for (r, c), (node, t) in sorted(b.items()):
+        outs = [(t, n) for n in range(num_nodes) if n != node]
+        for n in range(num_nodes):
+            if n != node:
+                to_del[n].append(t)
+        cmds.add_move((t, node), outs)
+    products = {}
+    for i in range(split):
+        for j in range(split):
+            for k in range(split):
+                target = i % num_nodes
+                cmds.add_apply("matrix_mult", ["dense", "dense"], ["dense"], gpu_mult,
+                               [(a[i, k][1], target), (b[k, j][1], target)], [(tid, target)])
+                to_del[target].append(tid)
+                products.setdefault((i, j), []).append((tid, target))
+                tid += 1
+    c_tids = {}
+    for r in range(split):
+        for c in range(split):
+            target = (r + c * split) % num_nodes
+            cmds.add_reduce("matrix_add", ["dense", "dense"], ["dense"], gpu_add,
+                            [(t, target) for t, _ in products[r, c]], (tid, target))
+            c_tids[r, c] = tid
+            tid += 1
+    for n in range(num_nodes):
+        cmds.add_delete([(t, n) for t in to_del[n]])
+    return cmds, {k: v[1] for k, v in a.items()}, {k: v[1] for k, v in b.items()}, c_tids

@@ -13,6 +13,7 @@
 // node and no host work.
 #include "gemm_bf16.cuh"
 
+#include <algorithm>
 #include <cmath>
 #include <tuple>
 
@@ -857,15 +858,51 @@ int bbts_hip_tune_batch(bbts_batch *b, int reps, float *times_us) {
     float best_us = 0.0f;
     std::vector<float> all((size_t)kNumCfgs, -1.0f);
     const bool fast = fast_ok(view_of(d), d.dtype == 1 ? 8 : 4);
+    // The timed launch holds R copies of the op, enough to fill the chip, each with operands
+    // and output of its own: copies of one shape sharing A and B would read them from the
+    // caches, while the stream's ops read theirs from HBM.
+    const size_t esz = d.dtype == 1 ? 2 : 4;
+    const size_t bytesA = ((size_t)(d.M - 1) * (size_t)d.lda + (size_t)d.K) * esz;
+    const size_t bytesB = d.K > 0 ? ((size_t)(d.K - 1) * (size_t)d.ldb + (size_t)d.N) * esz : 0;
+    const size_t bytesC = ((size_t)(d.M - 1) * (size_t)d.ldc + (size_t)d.N) * 4;
+    auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+    const size_t per_copy = up(bytesA) + up(bytesB) + up(bytesC);
+    long long Rmax = 1;
     for (int c = 0; c < kNumCfgs; ++c) {
+      long long tiles = tiles_of(d, c);
+      long long R = (4LL * b->ctx->num_cus * occupancy(c, variant) + tiles - 1) / tiles;
+      if (R > Rmax) Rmax = R;
+    }
+    // at most 256 copies and 1 GiB of copies per shape
+    Rmax = std::min<long long>(Rmax, 256);
+    Rmax = std::max<long long>(1, std::min<long long>(Rmax, (long long)((size_t(1) << 30) / std::max<size_t>(per_copy, 1))));
+    void *copies = nullptr;
+    size_t copies_rounded = 0;
+    if ((rc = pool_alloc(b->ctx, per_copy * (size_t)Rmax, &copies, &copies_rounded))) break;
+    std::vector<bbts_gemm_desc_t> reps_desc((size_t)Rmax, d);
+    for (long long r = 0; r < Rmax && rc == BBTS_OK; ++r) {
+      char *base = static_cast<char *>(copies) + (size_t)r * per_copy;
+      bbts_gemm_desc_t &x = reps_desc[(size_t)r];
+      x.A = base;
+      x.B = base + up(bytesA);
+      x.C = reinterpret_cast<float *>(base + up(bytesA) + up(bytesB));
+      hipError_t e1 = hipMemcpyAsync(const_cast<void *>(x.A), d.A, bytesA, hipMemcpyDeviceToDevice, s);
+      hipError_t e2 = bytesB ? hipMemcpyAsync(const_cast<void *>(x.B), d.B, bytesB, hipMemcpyDeviceToDevice, s)
+                             : hipSuccess;
+      if (e1 != hipSuccess || e2 != hipSuccess) {
+        set_error("tuning copy failed: %s", hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+        rc = BBTS_E_HIP;
+      }
+    }
+    for (int c = 0; c < kNumCfgs && rc == BBTS_OK; ++c) {
       // configs that do not fit LDS, or that the element-wise path maps elsewhere, are not timed
       if (!valid_cfg(c, variant) || (!fast && generic_cfg(c) != c)) continue;
       long long tiles = tiles_of(d, c);
       long long target = 4LL * b->ctx->num_cus * occupancy(c, variant);
       long long R = (target + tiles - 1) / tiles;
       if (R < 1) R = 1;
-      if (R > 256) R = 256;
-      tmp.ops.assign((size_t)R, d);
+      if (R > Rmax) R = Rmax;
+      tmp.ops.assign(reps_desc.begin(), reps_desc.begin() + R);
       tmp.prec.assign((size_t)R, prec);
       tmp.planned = false;
       if ((rc = bbts_hip_batch_plan(&tmp, BBTS_SCHED_FORCED_CFG0 + c, nullptr))) break;
@@ -891,6 +928,8 @@ int bbts_hip_tune_batch(bbts_batch *b, int reps, float *times_us) {
         best_us = us;
       }
     }
+    (void)hipStreamSynchronize(s);
+    pool_release(b->ctx, copies, copies_rounded);
     if (rc) break;
     std::lock_guard<std::mutex> g(b->ctx->mu);
     b->ctx->tune_table[kv.first] = tune_val{best, best_us, all};

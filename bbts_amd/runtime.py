@@ -35,11 +35,24 @@ _SIGS = {
     "bbts_rt_reduce": [_vp, _i32, _i32, _i32, _pi32, ctypes.c_int, _i32, _pu32, ctypes.c_int],
     "bbts_rt_broadcast": [_vp, _i32, _i32, _pi32, ctypes.c_int],
     "bbts_rt_delete": [_vp, _i32, _pi32, ctypes.c_int],
+    "bbts_rt_parse_commands": [_vp, ctypes.c_size_t, ctypes.c_int, _pi64],
+    "bbts_rt_load_commands": [_vp, _vp, ctypes.c_size_t, ctypes.c_int, _i32, _pi64],
     "bbts_rt_run": [_vp, ctypes.c_int, ctypes.c_int, _vp],
     "bbts_rt_stats": [_vp] + [_pi64] * 4,
     "bbts_rt_retire_order": [_vp, _pi32, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
 }
 _lib = None
+
+BINARY, TEXT = 0, 1
+COMMAND_TYPES = ("APPLY", "REDUCE", "MOVE", "DELETE")
+
+
+def parse_commands(data, fmt=BINARY):
+    """Parse a reference command file without running it: commands per type."""
+    buf = ctypes.create_string_buffer(bytes(data), len(data))
+    c = (ctypes.c_int64 * 4)()
+    _check(rtlib().bbts_rt_parse_commands(buf, len(data), fmt, c))
+    return dict(zip(COMMAND_TYPES, list(c)))
 
 
 def declared_symbols():
@@ -169,6 +182,15 @@ class Runtime:
         cid = self._id()
         _check(rtlib().bbts_rt_delete(self.h, cid, t, n))
         return cid
+
+    def load_commands(self, data, fmt=BINARY):
+        """Queue a reference command file (bytes; BINARY .bbts or TEXT .sbbts) for this GPU.
+        Returns the commands queued per type."""
+        buf = ctypes.create_string_buffer(bytes(data), len(data))
+        q = (ctypes.c_int64 * 4)()
+        _check(rtlib().bbts_rt_load_commands(self.h, buf, len(data), fmt, self._next_cmd + 1, q))
+        self._next_cmd += sum(q)
+        return dict(zip(COMMAND_TYPES, list(q)))
 
     def run(self, policy=SERIAL_LIFO, precision=0, costmodel=None):
         _check(rtlib().bbts_rt_run(self.h, policy, precision, costmodel.h if costmodel is not None else None))

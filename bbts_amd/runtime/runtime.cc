@@ -4,6 +4,7 @@
 #include <cmath>
 
 #include "bbts_rt.h"
+#include "command_file.h"
 
 namespace bbts {
 
@@ -875,6 +876,83 @@ int bbts_rt_delete(bbts_rt *rt, int32_t cmd_id, const int32_t *tids, int n) {
   c.type = command_t::DELETE;
   c.inputs.assign(tids, tids + n);
   return queue(rt, std::move(c));
+}
+
+int bbts_rt_parse_commands(const void *data, size_t len, int format, int64_t *counts) {
+  if (!data || !counts || (format != 0 && format != 1)) return BBTS_E_INVALID;
+  return guarded([&] {
+    auto cmds = format == 0 ? parse_bbts_binary(static_cast<const uint8_t *>(data), len)
+                            : parse_bbts_text(static_cast<const char *>(data), len);
+    for (int t = 0; t < 4; ++t) counts[t] = 0;
+    for (auto &c : cmds) ++counts[c.type];
+  });
+}
+
+// One GPU runs every node's share of the file.  A tensor id names one block here, created on
+// its home node (the node of the APPLY / REDUCE that outputs it); MOVEs of a tid to other
+// nodes are already satisfied, a DELETE of a tid on a node other than its home drops that
+// node's copy and so has nothing to do, and the home node's DELETE frees the block.  uniform
+// APPLYs without the optional seed word (the reference seeds with time()) are seeded with
+// their output tid.
+int bbts_rt_load_commands(bbts_rt *rt, const void *data, size_t len, int format, int32_t first_cmd_id,
+                          int64_t *queued) {
+  if (!rt || !data || (format != 0 && format != 1)) return BBTS_E_INVALID;
+  return guarded([&] {
+    auto cmds = format == 0 ? parse_bbts_binary(static_cast<const uint8_t *>(data), len)
+                            : parse_bbts_text(static_cast<const char *>(data), len);
+    int64_t counts[4] = {0, 0, 0, 0};
+    std::set<tid_t> deleted;
+    std::unordered_map<tid_t, int32_t> home;
+    int32_t id = first_cmd_id;
+    for (auto &fc : cmds) {
+      command_t c;
+      if (fc.type == command_t::APPLY || fc.type == command_t::REDUCE) {
+        ud_impl_t *impl = nullptr;
+        if (fc.gpu_unspecified) {
+          impl = rt->udm->find_match(fc.ud_name, fc.input_types, fc.output_types, true);
+          if (!impl) impl = rt->udm->find_match(fc.ud_name, fc.input_types, fc.output_types, false);
+        } else {
+          impl = rt->udm->find_match(fc.ud_name, fc.input_types, fc.output_types, fc.is_gpu);
+        }
+        if (!impl) throw std::runtime_error("command file: no implementation of '" + fc.ud_name + "' for these types");
+        c.type = (command_t::op_type_t)fc.type;
+        c.fun_id = impl->impl_id;
+        c.inputs = fc.inputs;
+        c.outputs = fc.outputs;
+        std::vector<uint32_t> w = fc.params;
+        if (fc.ud_name == "uniform" && w.size() == 4 && fc.outputs.size() == 1) w.push_back((uint32_t)fc.outputs[0]);
+        c.params = to_params(w.data(), (int)w.size());
+        if (c.type == command_t::REDUCE && (c.inputs.empty() || c.outputs.size() != 1))
+          throw std::runtime_error("command file: a REDUCE needs inputs and exactly one output");
+        for (size_t o = 0; o < fc.outputs.size(); ++o) home.emplace(fc.outputs[o], fc.output_nodes[o]);
+      } else if (fc.type == command_t::MOVE) {
+        if (fc.inputs.size() != 1) throw std::runtime_error("command file: a MOVE has exactly one input");
+        std::vector<tid_t> copies;
+        for (tid_t t : fc.outputs)
+          if (t != fc.inputs[0] && std::find(copies.begin(), copies.end(), t) == copies.end()) copies.push_back(t);
+        if (copies.empty()) continue;  // the same block on another node: already here
+        c.type = command_t::MOVE;
+        c.inputs = fc.inputs;
+        c.outputs = copies;
+      } else {
+        std::vector<tid_t> tids;
+        for (size_t q = 0; q < fc.inputs.size(); ++q) {
+          const tid_t t = fc.inputs[q];
+          auto h = home.find(t);
+          if (h != home.end() && h->second != fc.input_nodes[q]) continue;  // another node's copy
+          if (deleted.insert(t).second) tids.push_back(t);
+        }
+        if (tids.empty()) continue;
+        c.type = command_t::DELETE;
+        c.inputs = tids;
+      }
+      c.id = id++;
+      rt->rs.queue_command(c, [&](tid_t t) { return rt->storage->has(t); });
+      ++counts[c.type];
+    }
+    if (queued)
+      for (int t = 0; t < 4; ++t) queued[t] = counts[t];
+  });
 }
 
 int bbts_rt_run(bbts_rt *rt, int policy, int precision, bbts_costmodel *cm) {

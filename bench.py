@@ -434,9 +434,19 @@ def main():
             b.tune(reps=2)
         b.plan(pol, cm if pol == 3 else None)
         b.capture(sh)
-        ms = time_passes(torch, lambda: b.replay(sh), args.steps, args.warmup, s)
-        results[name] = {"ms": round(ms, 4), "tflops": round(total_flops / ms / 1e9, 2)}
         batches[name] = b
+    # Two rounds over all plans, each plan's faster round kept: the first plan timed in a
+    # process runs a few percent slow (the same plan timed again later is faster), so a
+    # single round would charge that to whichever plan goes first.
+    best = {}
+    for rnd in range(2):
+        for name, _ in policies:
+            b = batches[name]
+            ms = time_passes(torch, lambda: b.replay(sh), args.steps, args.warmup, s)
+            best[name] = min(ms, best.get(name, ms))
+    for name, _ in policies:
+        ms = best[name]
+        results[name] = {"ms": round(ms, 4), "tflops": round(total_flops / ms / 1e9, 2)}
         print("%s: %.3f ms  %.1f TFLOP/s" % (name, ms, total_flops / ms / 1e9), file=sys.stderr, flush=True)
 
     # re-run the learned graph last so C holds its output, then measure each grouped launch alone

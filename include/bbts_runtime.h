@@ -63,6 +63,22 @@ int bbts_rt_reduce(bbts_rt *rt, int32_t cmd_id, int32_t ud_id, int32_t impl_id, 
 int bbts_rt_broadcast(bbts_rt *rt, int32_t cmd_id, int32_t src, const int32_t *dst, int n);
 int bbts_rt_delete(bbts_rt *rt, int32_t cmd_id, const int32_t *tids, int n);
 
+/* ---- the reference's command files ---------------------------------------- */
+/* format 0: binary .bbts, parsed_command_list_t::serialize (src/commands/parsed_command.h:194-383,
+ * including the libstdc++ byte layout of std::tuple<tid_t, node_id_t>: node, then tid);
+ * format 1: text .sbbts, compile_source_file_t (src/commands/command_compiler.h:391-501, 977-1027).
+ * Parse only (no GPU needed): counts[4] = APPLY, REDUCE, MOVE, DELETE commands in the file. */
+int bbts_rt_parse_commands(const void *data, size_t len, int format, int64_t *counts);
+/* Parse and queue a file's commands for this GPU, with ids first_cmd_id, first_cmd_id + 1, ...
+ * This GPU runs every node's share: a tid is one block, homed on the node of the command
+ * that outputs it; MOVEs of a tid to other nodes are already satisfied and skipped, a DELETE
+ * on a node other than the tid's home (that node's copy) is skipped, and uniform APPLYs
+ * without a seed word are seeded with their output tid.  Implementations are matched by ud
+ * name, types and is_gpu (text files: the GPU impl when there is one).  queued[4] (may be
+ * NULL) = commands queued per type. */
+int bbts_rt_load_commands(bbts_rt *rt, const void *data, size_t len, int format, int32_t first_cmd_id,
+                          int64_t *queued);
+
 /* Run every queued command; precision applies to matrix_mult (BBTS_PREC_*); cm may be NULL. */
 int bbts_rt_run(bbts_rt *rt, int policy, int precision, bbts_costmodel *cm);
 

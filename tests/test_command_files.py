@@ -1,0 +1,63 @@
+"""The reference's command-file formats on the host (no GPU): the binary .bbts written as
+parsed_command_list_t::serialize writes it (src/commands/parsed_command.h:194-383) and the
+text .sbbts of compile_source_file_t (src/commands/command_compiler.h:391-501, 977-1027),
+parsed by libbbts_runtime.so; malformed files are errors, not crashes."""
+import struct
+
+import pytest
+
+from bbts_amd.commands import ParsedCommandList, SourceFile, generate_bmm
+from bbts_amd.runtime import BINARY, TEXT, parse_commands
+from bbts_amd import BBTSError
+
+
+def test_minimal_binary_bytes():
+    # one APPLY of "uniform" writing tid 7 on node 2, params {4u, 4u, 0.0f, 1.0f}
+    c = ParsedCommandList()
+    c.add_apply("uniform", [], ["dense"], False, [], [(7, 2)], [4, 4, 0.0, 1.0])
+    raw = c.serialize()
+    want = (struct.pack("<Q", 1) + struct.pack("<i", 0) + struct.pack("<Q", 7) + b"uniform" + b"\x00" +
+            struct.pack("<Q", 0) + struct.pack("<Q", 1) + struct.pack("<Q", 5) + b"dense" +
+            struct.pack("<Q", 0) + struct.pack("<Q", 1) + struct.pack("<ii", 2, 7) +
+            struct.pack("<Q", 4) + struct.pack("<IIff", 4, 4, 0.0, 1.0))
+    assert raw == want
+    assert parse_commands(raw) == {"APPLY": 1, "REDUCE": 0, "MOVE": 0, "DELETE": 0}
+
+
+@pytest.mark.parametrize("nodes", [1, 2, 3])
+def test_bmm_stream_counts(nodes):
+    split = 3
+    cmds, a, b, c = generate_bmm(split, nodes, 64)
+    got = parse_commands(cmds.serialize())
+    moves_a = sum(1 for (r, col) in a if (r * split + col) % nodes != r % nodes)
+    assert got == {"APPLY": 2 * split * split + split ** 3, "REDUCE": split * split,
+                   "MOVE": moves_a + split * split, "DELETE": nodes}
+
+
+def test_malformed_binary_is_an_error():
+    cmds, *_ = generate_bmm(2, 2, 32)
+    raw = cmds.serialize()
+    for bad in (raw[:-1], raw + b"\x00", raw[:40], b"", struct.pack("<Q", 10 ** 9)):
+        with pytest.raises(BBTSError):
+            parse_commands(bad)
+    bad_type = bytearray(raw)
+    bad_type[8:12] = struct.pack("<i", 9)
+    with pytest.raises(BBTSError):
+        parse_commands(bytes(bad_type))
+
+
+def test_text_source_file():
+    f = SourceFile()
+    f.add_function(0, "uniform", [], ["dense"])
+    f.add_function(1, "matrix_mult", ["dense", "dense"], ["dense"])
+    f.add_function(2, "matrix_add", ["dense", "dense"], ["dense"])
+    f.add_command(0, "APPLY", [], [0], [16, 16, 0.0, 1.0])
+    f.add_command(0, "APPLY", [], [1], [16, 16, 0.0, 1.0])
+    f.add_command(1, "APPLY", [0, 1], [2])
+    f.add_command(2, "REDUCE", [2, 0], [3])
+    f.add_command(-1, "DELETE", [2], [])
+    assert parse_commands(f.text().encode(), TEXT) == {"APPLY": 3, "REDUCE": 1, "MOVE": 0, "DELETE": 1}
+    for bad in ("1\n0 uniform 0 1 dense\n1\n0 APPLY 0 1 5 1 double 2.0\n", "0\n1\n7 APPLY 0 1 5 0\n",
+                "0\n1\n0 SEND 0 0 0\n", "1\n0 uniform 0 1 dense\n2\n0 APPLY 0 1 5 0\n"):
+        with pytest.raises(BBTSError):
+            parse_commands(bad.encode(), TEXT)

@@ -1,0 +1,72 @@
+"""A reference command file run end to end on the GPU: the bmm application's stream
+(applications/bmm/generate_bmm.cc) written as the reference's binary .bbts for 1, 2 and 3
+nodes, and as the text .sbbts, loaded into the runtime and run on this GPU.
+
+Bar: the uniform blocks bit-exact with the oracle generator (seeded with their tids), every
+C block against the FP64 product of the generated blocks (<= 2e-6 relative Frobenius), the
+reduce outputs' metas byte-exact, and the end state the reference's home copies: A, B and C
+kept, every product and every other node's copy deleted.
+"""
+import numpy as np
+import pytest
+
+from bbts_amd.commands import SourceFile, generate_bmm
+from bbts_amd.runtime import BINARY, TEXT, Runtime
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(rt, orc, split, bs, a, b, c):
+    A = {k: orc.uniform(bs * bs, t, 0.0, 1.0).reshape(bs, bs) for k, t in a.items()}
+    B = {k: orc.uniform(bs * bs, t, 0.0, 1.0).reshape(bs, bs) for k, t in b.items()}
+    for k, t in a.items():
+        assert np.array_equal(rt.get_dense(t), A[k])
+    for (i, j), t in c.items():
+        ref = sum(orc.gemm_f64(A[i, k], B[k, j]) for k in range(split))
+        got = rt.get_dense(t).astype(np.float64)
+        assert got.shape == (bs, bs)
+        assert np.linalg.norm(got - ref) <= 2e-6 * np.linalg.norm(ref), (i, j)
+    assert rt.num_tensors() == 3 * split * split
+
+
+@pytest.mark.parametrize("nodes", [1, 2, 3])
+def test_bmm_command_file(ctx, orc, nodes):
+    split, bs = 3, 96
+    cmds, a, b, c = generate_bmm(split, nodes, bs)
+    rt = Runtime(ctx)
+    q = rt.load_commands(cmds.serialize(), BINARY)
+    print("nodes %d: queued %s" % (nodes, q), flush=True)
+    assert q["APPLY"] == 2 * split * split + split ** 3 and q["REDUCE"] == split * split and q["MOVE"] == 0
+    rt.run()
+    _check(rt, orc, split, bs, a, b, c)
+    rt.close()
+
+
+def test_bmm_source_file(ctx, orc):
+    split, bs = 2, 64
+    f = SourceFile()
+    f.add_function(0, "uniform", [], ["dense"])
+    f.add_function(1, "matrix_mult", ["dense", "dense"], ["dense"])
+    f.add_function(2, "matrix_add", ["dense", "dense"], ["dense"])
+    a = {(i, j): i * split + j for i in range(split) for j in range(split)}
+    b = {(i, j): 100 + i * split + j for i in range(split) for j in range(split)}
+    for t in list(a.values()) + list(b.values()):
+        f.add_command(0, "APPLY", [], [t], [bs, bs, 0.0, 1.0])
+    c, temps = {}, []
+    for i in range(split):
+        for j in range(split):
+            prods = []
+            for k in range(split):
+                t = 1000 + 10 * (i * split + j) + k
+                f.add_command(1, "APPLY", [a[i, k], b[k, j]], [t])
+                prods.append(t)
+            c[i, j] = 2000 + i * split + j
+            f.add_command(2, "REDUCE", prods, [c[i, j]])
+            temps += prods
+    f.add_command(-1, "DELETE", temps, [])
+    rt = Runtime(ctx)
+    q = rt.load_commands(f.text().encode(), TEXT)
+    assert q == {"APPLY": 2 * split * split + split ** 3, "REDUCE": split * split, "MOVE": 0, "DELETE": 1}
+    rt.run()
+    _check(rt, orc, split, bs, a, b, c)
+    rt.close()
