@@ -141,7 +141,8 @@ inline bool fast_ok(const op_view &v, int epv) {
 // scalar (the soffset operand); per stage a lane only picks its offset or the out-of-range
 // one.  Global reads run two stages ahead of the MFMAs through two register sets: the
 // loads of stage k+2 are issued before stage k is computed, and stage k+1, loaded one
-// iteration earlier, is written to the free LDS buffer after it; one barrier per stage.
+// iteration earlier, is written to the free LDS buffer in shares between stage k's MFMA
+// steps; one barrier per stage.
 // The K loop runs in pairs of stages with no conditional loads or stores, so the wait
 // before an LDS write leaves the newest stage's loads in flight.
 // ---------------------------------------------------------------------------
@@ -211,12 +212,14 @@ __device__ __forceinline__ void tile_fast(const op_view &op, int m0, int n0, uin
     for (int r = 0; r < B_PER; ++r) rb[r] = bload16(rB, (brow + BRS * r < lim) ? vb[r] : kOOB, sb);
   };
 
-  // BF16 operands go to LDS as loaded; FP32 operands are split into S BF16 slice planes
-  auto store = [&](const uint4 (&ra)[A_PER], const uint4 (&rb)[B_PER], int buf) {
+  // BF16 operands go to LDS as loaded; FP32 operands are split into S BF16 slice planes.
+  // part / nparts: the share of the vectors (r = part mod nparts) written by this call.
+  auto store = [&](const uint4 (&ra)[A_PER], const uint4 (&rb)[B_PER], int buf, int part = 0, int nparts = 1) {
     uint16_t *as = As + buf * S * AE;
     uint16_t *bs = Bs + buf * S * BE;
 #pragma unroll
     for (int r = 0; r < A_PER; ++r) {
+      if (r % nparts != part) continue;
       const int row = arow + ARS * r;
       if constexpr (ES == 2) {
         *reinterpret_cast<uint4 *>(&as[sw<BK>(row, ac * 8)]) = ra[r];
@@ -236,6 +239,7 @@ __device__ __forceinline__ void tile_fast(const op_view &op, int m0, int n0, uin
     }
 #pragma unroll
     for (int r = 0; r < B_PER; ++r) {
+      if (r % nparts != part) continue;
       const int krow = brow + BRS * r;
       if constexpr (ES == 2) {
         *reinterpret_cast<uint4 *>(&bs[swn<BN>(krow, bc * 8)]) = rb[r];
@@ -264,7 +268,10 @@ __device__ __forceinline__ void tile_fast(const op_view &op, int m0, int n0, uin
       for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
 
   const int half = lane >> 5, col = lane & 31;
-  auto compute = [&](int buf) {
+  // compute the stage in LDS buffer `buf`; after each 16-deep step write a share of the next
+  // stage's registers (sa_/sb_) to LDS buffer `wbuf`, so the LDS writes go out between MFMAs
+  // instead of in one burst before the barrier (wbuf < 0: nothing to write)
+  auto compute = [&](int buf, const uint4 (&sa_)[A_PER], const uint4 (&sb_)[B_PER], int wbuf) {
     const uint16_t *as = As + buf * S * AE;
     const uint16_t *bs = Bs + buf * S * BE;
 #pragma unroll
@@ -303,6 +310,7 @@ __device__ __forceinline__ void tile_fast(const op_view &op, int m0, int n0, uin
             for (int j = 0; j < FN; ++j)
               acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[sa][i], bfr[sb][j], acc[i][j], 0, 0, 0);
         }
+      if (wbuf >= 0) store(sa_, sb_, wbuf, kk, BK / 16);
     }
   };
 
@@ -318,16 +326,14 @@ __device__ __forceinline__ void tile_fast(const op_view &op, int m0, int n0, uin
   for (; kt + 1 < nk; kt += 2) {
     load(a0, b0, (kt + 2) * BK);
     __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of this stage's LDS writes
-    compute(0);
-    store(a1, b1, 1);
+    compute(0, a1, b1, 1);
     __syncthreads();
     load(a1, b1, (kt + 3) * BK);
     __builtin_amdgcn_sched_barrier(0);
-    compute(1);
-    store(a0, b0, 0);
+    compute(1, a0, b0, 0);
     __syncthreads();
   }
-  if (kt < nk) compute(0);  // odd stage count: the last stage sits in buffer 0
+  if (kt < nk) compute(0, a1, b1, -1);  // odd stage count: the last stage sits in buffer 0
 
   // epilogue through a descriptor over C: elements outside the M x N block are dropped
   const __amdgpu_buffer_rsrc_t rC = make_rsrc(op.C, ((int64_t)(M - 1) * op.ldc + N) * 4);

@@ -242,13 +242,14 @@ int heuristic_cfg(int M, int N, int K) {
 // 512^3 ... 4096^3, 1152x1152x2048 and 9216x9216x4096: up to a quarter tile per CU the
 // 64x64 tile wins (four workgroups on every CU), up to one tile per CU the 64x128 tile,
 // and beyond that the 128x128 tile. The 6-product split, whose 128x128 tile is limited to
-// one workgroup per CU by its LDS, stays on 64x128 from a half tile per CU up.
+// one workgroup per CU by its LDS, takes 64x128 from a half tile per CU up and 256x128 from
+// four tiles per CU up.
 // Configs: 0 256x128, 1 128x128, 2 128x64, 3 64x64, 4 64x128.
 int single_op_cfg(int M, int N, int K, int batch, int variant, int num_cus) {
   (void)K;
   const long long t128 = (long long)((M + 127) / 128) * ((N + 127) / 128) * batch;
   const long long cus = num_cus > 0 ? num_cus : 256;
-  if (variant == 3) return t128 < cus / 2 ? 3 : 4;
+  if (variant == 3) return t128 < cus / 2 ? 3 : (t128 < 4 * cus ? 4 : 0);
   return t128 <= cus / 4 ? 3 : (t128 < cus ? 4 : 1);
 }
 
