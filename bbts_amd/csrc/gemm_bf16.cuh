@@ -44,10 +44,15 @@ struct stage_k {
   static constexpr int value = (S == 1 && BM * BN <= 128 * 128) ? 64 : 32;
 };
 
-// element index of (row, k) in a [rows][BK] BF16 tile with swizzled 16-byte chunks
+// element index of (row, k) in a [rows][BK] BF16 tile with swizzled 16-byte chunks.  The
+// LDS serves 256 bytes per cycle (64 banks) and a 16-byte-per-lane read 16 lanes at a time:
+// the 16 consecutive rows such a read touches (one chunk each) must land on 16 distinct
+// 16-byte bank slots.  A 128-byte row (BK = 64) fills half a bank line, so rows of equal
+// parity share a half and need 8 distinct chunk positions: XOR by (row / 2) mod 8.  A 64-byte
+// row (BK = 32) fills a quarter: rows equal mod 4 need 4 positions: XOR by (row / 4) mod 4.
 template <int BK>
 __device__ __forceinline__ int sw(int row, int k) {
-  const int x = BK == 64 ? (row & 7) : ((row >> 1) & 3);
+  const int x = BK == 64 ? ((row >> 1) & 7) : ((row >> 2) & 3);
   return row * BK + ((((k >> 3) ^ x)) << 3) + (k & 7);
 }
 

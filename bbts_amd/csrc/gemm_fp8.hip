@@ -150,10 +150,12 @@ struct fp8_view {
   uint32_t w0, w1;
 };
 
-// element index of (row, byte) in a [rows][RB] byte tile with XOR-swizzled 16-byte chunks
+// element index of (row, byte) in a [rows][RB] byte tile with XOR-swizzled 16-byte chunks,
+// laid out like the BF16 A tile: the 16 consecutive rows one 16-lane read phase touches land
+// on distinct 16-byte slots of the 256-byte bank line
 template <int RB>
 __device__ __forceinline__ int swb(int row, int byte) {
-  const int x = RB == 128 ? (row & 7) : ((row >> 1) & 3);
+  const int x = RB == 128 ? ((row >> 1) & 7) : ((row >> 2) & 3);
   return row * RB + ((((byte >> 4) ^ x)) << 4) + (byte & 15);
 }
 
