@@ -76,15 +76,32 @@ __global__ void zero_words_kernel(uint32_t *p, int n) {
 }
 
 // amax |X| over batch x rows x cols (row stride ld, batch stride bs), into *out (float bits)
+// Raise *out to m's bits (non-negative floats order as their bits) only when m beats the
+// value already there: after the first few blocks almost every block skips the atomic, so
+// the blocks do not serialise on one address.  A stale read only costs an extra atomic.
+__device__ __forceinline__ void amax_publish(uint32_t *out, float m) {
+  if (m > 0.0f && m > __uint_as_float(__hip_atomic_load(out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+    atomicMax(out, __float_as_uint(m));
+}
+
+// Rows blockIdx.y, +gridDim.y, ...; four rows' loads in flight per step.
 __global__ __launch_bounds__(256) void amax_kernel(const float *__restrict__ X, int rows, int cols, int64_t ld,
                                                    int64_t bs, uint32_t *out) {
   __shared__ float red[4];
   const float *x = X + (int64_t)blockIdx.z * bs;
+  const int gy = (int)gridDim.y;
   float m = 0.0f;
-  for (int r = blockIdx.y; r < rows; r += gridDim.y)
-    for (int c = blockIdx.x * 256 + threadIdx.x; c < cols; c += gridDim.x * 256) m = fmaxf(m, fabsf(x[(int64_t)r * ld + c]));
+  for (int c = blockIdx.x * 256 + threadIdx.x; c < cols; c += gridDim.x * 256) {
+    int r = blockIdx.y;
+    for (; r + 3 * gy < rows; r += 4 * gy) {
+      const float v0 = x[(int64_t)r * ld + c], v1 = x[(int64_t)(r + gy) * ld + c];
+      const float v2 = x[(int64_t)(r + 2 * gy) * ld + c], v3 = x[(int64_t)(r + 3 * gy) * ld + c];
+      m = fmaxf(m, fmaxf(fmaxf(fabsf(v0), fabsf(v1)), fmaxf(fabsf(v2), fabsf(v3))));
+    }
+    for (; r < rows; r += gy) m = fmaxf(m, fabsf(x[(int64_t)r * ld + c]));
+  }
   m = block_max(m, red);
-  if (threadIdx.x == 0 && m > 0.0f) atomicMax(out, __float_as_uint(m));
+  if (threadIdx.x == 0) amax_publish(out, m);
 }
 
 // One 64 x 64 source tile -> e4m3 codes in destination orientation (TRANS: dest row = source
@@ -102,14 +119,19 @@ __global__ __launch_bounds__(256) void quant_kernel(const float *__restrict__ X,
   const int e1 = scale_exp(sc[0]);
   const int e2 = LO ? scale_exp(sc[1]) : 0;
   float rmax = 0.0f;
-  for (int i = threadIdx.x; i < 64 * 64; i += 256) {
-    const int sr = i >> 6, scl = i & 63;
-    const int gr = r0 + sr, gc = c0 + scl;
-    const float v = (gr < rows && gc < cols) ? x[(int64_t)gr * ld + gc] : 0.0f;
-    const uint32_t qh = e4m3_rne(ldexpf(v, -e1));
+  float v[16];  // all 16 loads issued before any is used
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int i = threadIdx.x + j * 256, gr = r0 + (i >> 6), gc = c0 + (i & 63);
+    v[j] = (gr < rows && gc < cols) ? x[(int64_t)gr * ld + gc] : 0.0f;
+  }
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int i = threadIdx.x + j * 256, sr = i >> 6, scl = i & 63;
+    const uint32_t qh = e4m3_rne(ldexpf(v[j], -e1));
     uint32_t q = qh;
     if (LO || want_lo_amax) {
-      const float res = v - ldexpf(e4m3_dec(qh), e1);  // exact
+      const float res = v[j] - ldexpf(e4m3_dec(qh), e1);  // exact
       if (LO) q = e4m3_rne(ldexpf(res, -e2));
       else rmax = fmaxf(rmax, fabsf(res));
     }
@@ -118,7 +140,7 @@ __global__ __launch_bounds__(256) void quant_kernel(const float *__restrict__ X,
   }
   if (!LO && want_lo_amax) {
     rmax = block_max(rmax, red);
-    if (threadIdx.x == 0 && rmax > 0.0f) atomicMax(&sc[1], __float_as_uint(rmax));
+    if (threadIdx.x == 0) amax_publish(&sc[1], rmax);
   }
   __syncthreads();
   if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0) se[LO ? 1 : 0] = LO ? e2 : e1;
@@ -330,7 +352,9 @@ int quantize(const float *X, int rows, int cols, int64_t ld, int64_t bs, int bat
              uint8_t *Q, int64_t ldq, int64_t qbs, int64_t plane, uint32_t *sc, int *se, hipStream_t s) {
   {
     const int gx = (cols + 255) / 256 < 64 ? (cols + 255) / 256 : 64;
-    const int gy = rows < 1024 ? rows : 1024;
+    // about 1024 workgroups in all, each thread reading >= 4 rows where the block is tall enough
+    int gy = 1024 / (gx * (batch < 1024 ? batch : 1024));
+    gy = gy < 1 ? 1 : (gy < rows ? gy : rows);
     hipLaunchKernelGGL(amax_kernel, dim3((unsigned)gx, (unsigned)gy, (unsigned)batch), dim3(256), 0, s, X, rows, cols,
                        ld, bs, sc);
     BBTS_LAUNCH_CHECK();
