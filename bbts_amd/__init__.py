@@ -72,6 +72,16 @@ _SIGS = {
     "bbts_hip_contraction": [_vp, _vp, _vp, _vp, ctypes.POINTER(_i32), ctypes.c_int, ctypes.c_int, _vp],
     "bbts_hip_contraction_raw": [_vp, ctypes.POINTER(_i32), ctypes.c_int, ctypes.c_int, ctypes.POINTER(_i64), _vp,
                                  ctypes.c_int, ctypes.POINTER(_i64), _vp, _vp, ctypes.c_int, _vp],
+    "bbts_hip_cu_ew": [_vp, _vp, _vp, ctypes.POINTER(_i32), ctypes.c_int, _vp],
+    "bbts_hip_cu_ew_raw": [_vp, ctypes.POINTER(_i32), ctypes.c_int, ctypes.c_int, ctypes.POINTER(_i64), _vp, _vp, _vp],
+    "bbts_hip_cu_ewb": [_vp, _vp, _vp, _vp, ctypes.POINTER(_i32), ctypes.c_int, _vp],
+    "bbts_hip_cu_ewb_raw": [_vp, ctypes.POINTER(_i32), ctypes.c_int, ctypes.c_int, ctypes.POINTER(_i64), _vp,
+                            ctypes.c_int, ctypes.POINTER(_i64), _vp, _vp, _vp],
+    "bbts_hip_cu_ewb_castable": [_vp, _vp, _vp, _vp, ctypes.POINTER(_i32), ctypes.c_int, _vp],
+    "bbts_hip_cu_ewb_castable_raw": [_vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_size_t, _vp],
+    "bbts_hip_cu_reduction": [_vp, _vp, _vp, ctypes.POINTER(_i32), ctypes.c_int, _vp],
+    "bbts_hip_cu_reduction_raw": [_vp, ctypes.POINTER(_i32), ctypes.c_int, ctypes.c_int, ctypes.POINTER(_i64), _vp,
+                                  _vp, ctypes.c_int, _vp],
     "bbts_hip_gemm": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _i64, ctypes.c_int, _vp, _i64,
                       ctypes.c_int, _vp, _i64, _f, ctypes.c_int, _vp],
     "bbts_hip_gemm_batched": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _i64, _i64, ctypes.c_int, _vp,

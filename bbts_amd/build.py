@@ -39,7 +39,13 @@ def _run(cmd):
     return out
 
 
+# per-source flags after HIP_FLAGS: the cutensor elementwise kernels round alpha * lhs and
+# beta * rhs on their own before the binary op, as the reference's separate MKL calls do
+PER_SOURCE_FLAGS = {"cutensor_ew.hip": ["-ffp-contract=off"]}
+
+
 def _compile(src, extra=()):
+    extra = list(extra) + PER_SOURCE_FLAGS.get(os.path.basename(src), [])
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
     deps = [src] + glob.glob(os.path.join(os.path.dirname(src), "*.h")) + \
         glob.glob(os.path.join(os.path.dirname(src), "*.cuh")) + [os.path.join(ROOT, "include", "bbts_hip.h")]

@@ -138,6 +138,9 @@ struct bbts_hip_ctx {
   // small device scratch used by kernels that need work counters (one per stream use)
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
+  // grow-only per-stream buffers (partials of split reductions); outgrown ones are kept
+  // until shutdown so that graphs captured with them stay valid
+  std::map<uintptr_t, std::vector<std::pair<void *, size_t>>> stream_scratch;
 };
 
 namespace bbts_hip {

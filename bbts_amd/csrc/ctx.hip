@@ -165,6 +165,8 @@ int bbts_hip_shutdown(bbts_hip_ctx *ctx) {
     if (kv.second.base) (void)hipFree(kv.second.base);
   for (auto &kv : ctx->free_pool) (void)hipFree(kv.second);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
+  for (auto &kv : ctx->stream_scratch)
+    for (auto &buf : kv.second) (void)hipFree(buf.first);
   delete ctx;
   return BBTS_OK;
 }

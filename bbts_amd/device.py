@@ -163,6 +163,42 @@ class Context:
         call("bbts_hip_contraction", self.h, lhs.ptr, rhs.ptr, out.ptr,
              p.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(p), precision, stream)
 
+    # ---- cutensor elementwise / reduction family (parameter words: ew_params etc. below)
+    def cu_ew(self, inp, out, params, stream=None):
+        p = _words(params)
+        call("bbts_hip_cu_ew", self.h, inp.ptr, out.ptr, _i32p(p), len(p), stream)
+
+    def cu_ewb(self, lhs, rhs, out, params, stream=None):
+        p = _words(params)
+        call("bbts_hip_cu_ewb", self.h, lhs.ptr, rhs.ptr, out.ptr, _i32p(p), len(p), stream)
+
+    def cu_ewb_castable(self, lhs, rhs, out, op, stream=None):
+        p = _words([op])
+        call("bbts_hip_cu_ewb_castable", self.h, lhs.ptr, rhs.ptr, out.ptr, _i32p(p), len(p), stream)
+
+    def cu_reduction(self, inp, out, params, stream=None):
+        p = _words(params)
+        call("bbts_hip_cu_reduction", self.h, inp.ptr, out.ptr, _i32p(p), len(p), stream)
+
+    def cu_ew_raw(self, params, dims, inp, out, stream=None):
+        p, d = _words(params), _dims(dims)
+        call("bbts_hip_cu_ew_raw", self.h, _i32p(p), len(p), len(dims), _i64p(d), inp.data_ptr(), out.data_ptr(),
+             stream)
+
+    def cu_ewb_raw(self, params, dims_l, lhs, dims_r, rhs, out, stream=None):
+        p, dl, dr = _words(params), _dims(dims_l), _dims(dims_r)
+        call("bbts_hip_cu_ewb_raw", self.h, _i32p(p), len(p), len(dims_l), _i64p(dl), lhs.data_ptr(), len(dims_r),
+             _i64p(dr), rhs.data_ptr(), out.data_ptr(), stream)
+
+    def cu_ewb_castable_raw(self, op, lhs, rhs, out, stream=None):
+        call("bbts_hip_cu_ewb_castable_raw", self.h, op, lhs.data_ptr(), rhs.data_ptr(), out.data_ptr(), out.numel(),
+             stream)
+
+    def cu_reduction_raw(self, params, dims, inp, out, segments=0, stream=None):
+        p, d = _words(params), _dims(dims)
+        call("bbts_hip_cu_reduction_raw", self.h, _i32p(p), len(p), len(dims), _i64p(d), inp.data_ptr(),
+             out.data_ptr(), segments, stream)
+
     # ---- raw entry points on torch device tensors
     def gemm(self, A, B, C, M, N, K, lda, ldb, ldc, ta=False, tb=False, alpha=1.0, precision=0, stream=None):
         call("bbts_hip_gemm", self.h, M, N, K, A.data_ptr(), lda, int(ta), B.data_ptr(), ldb, int(tb),
@@ -180,6 +216,43 @@ class Context:
 
     def to_bf16(self, src, dst, stream=None):
         call("bbts_hip_f32_to_bf16", self.h, src.data_ptr(), dst.data_ptr(), src.numel(), stream)
+
+
+def _fbits(x):
+    return struct.unpack("<i", struct.pack("<f", float(x)))[0]
+
+
+def _words(params):
+    return np.ascontiguousarray(params, dtype=np.int32)
+
+
+def _dims(dims):
+    return np.ascontiguousarray(list(dims) + [0] * (4 - len(dims)), dtype=np.int64)
+
+
+def _i32p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def _i64p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+
+
+def ew_params(f, alpha, ordering, c=None):
+    """ew parameter words (libraries/cutensor/ew.h:21-29): f, [c when f == 6], alpha, ordering."""
+    w = [int(f)] + ([_fbits(c if c is not None else 0.0)] if f == 6 else []) + [_fbits(alpha)]
+    return w + [int(m) for m in ordering]
+
+
+def ewb_params(op, alpha, beta, ordering_l, ordering_r):
+    """ewb parameter words (libraries/cutensor/ewb.h:21-36)."""
+    return ([int(op), _fbits(alpha), _fbits(beta), len(ordering_l)] + [int(m) for m in ordering_l] +
+            [len(ordering_r)] + [int(m) for m in ordering_r])
+
+
+def reduction_params(op, alpha, ordering):
+    """reduction parameter words (libraries/cutensor/reduction.h:19-27)."""
+    return [int(op), _fbits(alpha)] + [int(m) for m in ordering]
 
 
 class GemmDesc(ctypes.Structure):

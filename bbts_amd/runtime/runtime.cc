@@ -186,6 +186,131 @@ struct hip_contraction_t : public ud_impl_t {
   }
 };
 
+// ---- the cutensor elementwise / reduction family (libraries/cutensor.cc:26-29)
+static uint64_t cu_elems(const cu_shape_t &m) {
+  uint64_t n = 1;
+  for (uint32_t r = 0; r < m.rank && r < 4; ++r) n *= (uint64_t)std::max<int64_t>(m.dims[r], 0);
+  return n;
+}
+static std::vector<int32_t> param_words(const tensor_params_t &p) {
+  std::vector<int32_t> w(p.num_parameters());
+  for (size_t i = 0; i < w.size(); ++i) w[i] = p._params[i].i;
+  return w;
+}
+// out mode r = input mode ordering[r] (ew.h:31-41, reduction.h:29-37); unknown modes give 0
+static void reorder_meta(const cu_shape_t &in, const std::vector<int32_t> &w, size_t first, cu_shape_t &out) {
+  out.rank = (uint32_t)std::min<size_t>(w.size() > first ? w.size() - first : 0, 4);
+  for (uint32_t r = 0; r < out.rank; ++r) {
+    const int32_t m = w[first + r];
+    out.dims[r] = (m >= 0 && (uint32_t)m < in.rank && m < 4) ? in.dims[m] : 0;
+  }
+}
+
+// ew (ew.h:360-412): one cutensor input, out meta reordered by the ordering words
+struct hip_cu_ew_t : public ud_impl_t {
+  hip_cu_ew_t() {
+    impl_name = ud_name = "ew";
+    inputTypes = {"cutensor"};
+    outputTypes = {"cutensor"};
+    inputInplace = {};
+    is_gpu = true;
+    fn = [](const tensor_params_t &p, const tensor_args_t &in, tensor_args_t &out) {
+      auto w = param_words(p);
+      check(bbts_hip_cu_ew(p.ctx, in.ptr(0), out.ptr(0), w.data(), (int)w.size(), p.stream));
+    };
+  }
+  size_t get_complexity_hint(const tensor_params_t &, const meta_args_t &in) override {
+    return cu_elems(in.get<0>().as<cu_meta_t>().m());
+  }
+  void get_out_meta(const tensor_params_t &p, const meta_args_t &in, meta_args_t &out) const override {
+    auto w = param_words(p);
+    const size_t first = (!w.empty() && w[0] == 6) ? 3 : 2;  // f, [c], alpha
+    reorder_meta(in.get<0>().as<cu_meta_t>().m(), w, first, out.get<0>().as<cu_meta_t>().m());
+  }
+};
+
+// ewb (ewb.h:454-503): out meta from the two label lists (set_out_meta, ewb.h:38-53)
+struct hip_cu_ewb_t : public ud_impl_t {
+  hip_cu_ewb_t() {
+    impl_name = ud_name = "ewb";
+    inputTypes = {"cutensor", "cutensor"};
+    outputTypes = {"cutensor"};
+    inputInplace = {};
+    is_gpu = true;
+    fn = [](const tensor_params_t &p, const tensor_args_t &in, tensor_args_t &out) {
+      auto w = param_words(p);
+      check(bbts_hip_cu_ewb(p.ctx, in.ptr(0), in.ptr(1), out.ptr(0), w.data(), (int)w.size(), p.stream));
+    };
+  }
+  static cu_shape_t shape(const tensor_params_t &p, const meta_args_t &in) {
+    auto w = param_words(p);
+    const auto &l = in.get<0>().as<cu_meta_t>().m();
+    const auto &r = in.get<1>().as<cu_meta_t>().m();
+    cu_shape_t o{};
+    if (w.size() < 4) return o;
+    const int rl = std::max(0, std::min(w[3], 4));
+    if (w.size() < (size_t)(5 + rl)) return o;
+    std::vector<int32_t> ol(w.begin() + 4, w.begin() + 4 + rl), orr(w.begin() + 5 + rl, w.end());
+    int ro = 0;
+    for (int m : ol) ro = std::max(ro, m + 1);
+    for (int m : orr) ro = std::max(ro, m + 1);
+    o.rank = (uint32_t)std::min(ro, 4);
+    for (size_t x = 0; x < ol.size() && x < l.rank; ++x)
+      if (ol[x] >= 0 && ol[x] < 4) o.dims[ol[x]] = l.dims[x];
+    for (size_t x = 0; x < orr.size() && x < r.rank; ++x)
+      if (orr[x] >= 0 && orr[x] < 4) o.dims[orr[x]] = r.dims[x];
+    return o;
+  }
+  size_t get_complexity_hint(const tensor_params_t &p, const meta_args_t &in) override {
+    return cu_elems(shape(p, in));
+  }
+  void get_out_meta(const tensor_params_t &p, const meta_args_t &in, meta_args_t &out) const override {
+    out.get<0>().as<cu_meta_t>().m() = shape(p, in);
+  }
+};
+
+// ewb_castable (ewb_castable.h:91-124): out meta is lhs's; out may alias an input
+struct hip_cu_ewb_castable_t : public ud_impl_t {
+  hip_cu_ewb_castable_t() {
+    impl_name = ud_name = "ewb_castable";
+    inputTypes = {"cutensor", "cutensor"};
+    outputTypes = {"cutensor"};
+    inputInplace = {0, 1};
+    is_gpu = true;
+    fn = [](const tensor_params_t &p, const tensor_args_t &in, tensor_args_t &out) {
+      auto w = param_words(p);
+      check(bbts_hip_cu_ewb_castable(p.ctx, in.ptr(0), in.ptr(1), out.ptr(0), w.data(), (int)w.size(), p.stream));
+    };
+  }
+  size_t get_complexity_hint(const tensor_params_t &, const meta_args_t &in) override {
+    return cu_elems(in.get<0>().as<cu_meta_t>().m());
+  }
+  void get_out_meta(const tensor_params_t &, const meta_args_t &in, meta_args_t &out) const override {
+    out.get<0>().as<cu_meta_t>().m() = in.get<0>().as<cu_meta_t>().m();
+  }
+};
+
+// reduction (reduction.h:251-290): out meta reordered by the ordering words after op, alpha
+struct hip_cu_reduction_t : public ud_impl_t {
+  hip_cu_reduction_t() {
+    impl_name = ud_name = "reduction";
+    inputTypes = {"cutensor"};
+    outputTypes = {"cutensor"};
+    inputInplace = {};
+    is_gpu = true;
+    fn = [](const tensor_params_t &p, const tensor_args_t &in, tensor_args_t &out) {
+      auto w = param_words(p);
+      check(bbts_hip_cu_reduction(p.ctx, in.ptr(0), out.ptr(0), w.data(), (int)w.size(), p.stream));
+    };
+  }
+  size_t get_complexity_hint(const tensor_params_t &, const meta_args_t &in) override {
+    return cu_elems(in.get<0>().as<cu_meta_t>().m());
+  }
+  void get_out_meta(const tensor_params_t &p, const meta_args_t &in, meta_args_t &out) const override {
+    reorder_meta(in.get<0>().as<cu_meta_t>().m(), param_words(p), 2, out.get<0>().as<cu_meta_t>().m());
+  }
+};
+
 }  // namespace
 
 // ============================================================================
@@ -273,12 +398,22 @@ void register_cutensor_library(tensor_factory_t &factory, udf_manager_t &udm) {
                          for (uint32_t r = 0; r < s.rank && r < 4; ++r) n *= (size_t)s.dims[r];
                          return sizeof(tensor_meta_t) + 4 * n;
                        }});
-  auto f = std::make_unique<ud_func_t>();
-  f->ud_name = "contraction";
-  f->num_in = 2;
-  f->num_out = 1;
-  udm.register_udf(std::move(f));
-  udm.register_udf_impl(std::make_unique<hip_contraction_t>());
+  // the plugin's registration order (libraries/cutensor.cc:25-30), without the host-only
+  // init / expand / dropout
+  auto reg = [&](const char *name, size_t nin, bool ass_com, ud_impl_ptr_t impl) {
+    auto f = std::make_unique<ud_func_t>();
+    f->ud_name = name;
+    f->is_ass = f->is_com = ass_com;
+    f->num_in = nin;
+    f->num_out = 1;
+    udm.register_udf(std::move(f));
+    udm.register_udf_impl(std::move(impl));
+  };
+  reg("contraction", 2, false, std::make_unique<hip_contraction_t>());
+  reg("reduction", 1, false, std::make_unique<hip_cu_reduction_t>());
+  reg("ew", 1, false, std::make_unique<hip_cu_ew_t>());
+  reg("ewb", 2, false, std::make_unique<hip_cu_ewb_t>());
+  reg("ewb_castable", 2, true, std::make_unique<hip_cu_ewb_castable_t>());  // ewb_castable.h:132-133
 }
 
 // ============================================================================

@@ -225,17 +225,21 @@ def suite(torch, ctx, steps):
         out.append(d)
         print(json.dumps(d), flush=True)
 
+    cap = torch.cuda.Stream()  # default priority
+    hc = ctypes.c_void_p(cap.cuda_stream)
+
     def tm_graph(calls, reps=steps):
         """The calls captured once into a graph and replayed: per-call time of the kernels
-        themselves, without the host's launch gaps."""
+        themselves, without the host's launch gaps.  They run once on the capture stream
+        first, so that per-stream scratch exists before capture starts."""
+        torch.cuda.synchronize()
         for c in calls:
-            c(sh)
+            c(hc)
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
-            h = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        with torch.cuda.graph(g, stream=cap):
             for c in calls:
-                c(h)
+                c(hc)
         torch.cuda.synchronize()
         ms = time_passes(torch, g.replay, reps, 2, s) / len(calls)
         del g
@@ -266,6 +270,38 @@ def suite(torch, ctx, steps):
     emit("broadcast 2048x2048 block -> 32", 33.0 * tb, "GB/s", tm_graph(calls), HBM_PEAK_GBS)
     for d in dsts + outs + pool[8:]:
         d.free()
+
+    # the cutensor elementwise / reduction family on 2048 x 2048 FP32 column-major tensors,
+    # cycled over distinct buffers like the dense kernels above
+    from bbts_amd.device import ew_params, ewb_params, reduction_params
+    X = [torch.empty(mn, dtype=torch.float32, device="cuda") for _ in range(12)]
+    for q, t in enumerate(X):
+        ctx.gen_normal(t, 300 + q)
+    Y = [torch.empty(mn, dtype=torch.float32, device="cuda") for _ in range(8)]
+    bias = torch.empty(n, dtype=torch.float32, device="cuda")
+    ctx.gen_normal(bias, 299)
+    sq = (n, n)
+    cases = [
+        ("cutensor ew relu 2048x2048", 8.0 * mn,
+         lambda h, i: ctx.cu_ew_raw(ew_params(3, 1.0, (0, 1)), sq, X[i], Y[i], h)),
+        ("cutensor ew square, modes (1,0) 2048x2048 (LDS-tiled transpose)", 8.0 * mn,
+         lambda h, i: ctx.cu_ew_raw(ew_params(2, 1.0, (1, 0)), sq, X[i], Y[i], h)),
+        ("cutensor ew exp, modes (2,3,0,1) of 64x64x32x32", 8.0 * mn,
+         lambda h, i: ctx.cu_ew_raw(ew_params(1, 1.0, (2, 3, 0, 1)), (64, 64, 32, 32), X[i], Y[i], h)),
+        ("cutensor ewb add, 2048x2048 + bias over mode 1", 8.0 * mn + 4.0 * n,
+         lambda h, i: ctx.cu_ewb_raw(ewb_params(0, 1.0, 1.0, (0, 1), (1,)), sq, X[i], (n,), bias, Y[i], h)),
+        ("cutensor ewb mul, 2048x2048 * transposed 2048x2048 (gather)", 12.0 * mn,
+         lambda h, i: ctx.cu_ewb_raw(ewb_params(3, 1.0, 1.0, (0, 1), (1, 0)), sq, X[i], sq, X[i + 4], Y[i], h)),
+        ("cutensor ewb_castable add 2048x2048", 12.0 * mn,
+         lambda h, i: ctx.cu_ewb_castable_raw(0, X[i], X[i + 4], Y[i], h)),
+        ("cutensor reduction sum over mode 1, 2048x2048 -> 2048 (split fold)", 4.0 * mn + 4.0 * n,
+         lambda h, i: ctx.cu_reduction_raw(reduction_params(0, 1.0, (0,)), sq, X[i], Y[i], 0, h)),
+        ("cutensor reduction sum over mode 0, 2048x2048 -> 2048 (wave per output)", 4.0 * mn + 4.0 * n,
+         lambda h, i: ctx.cu_reduction_raw(reduction_params(0, 1.0, (1,)), sq, X[i], Y[i], 0, h)),
+    ]
+    for name, nbytes, fn in cases:
+        emit(name, nbytes, "GB/s", tm_graph([lambda h, i=i, fn=fn: fn(h, i) for i in range(8)]), HBM_PEAK_GBS)
+    del X, Y, bias
     blocks = pool[:8]
     outb = ctx.dense(n, n)
 

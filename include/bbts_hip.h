@@ -122,6 +122,41 @@ int bbts_hip_contraction_raw(bbts_hip_ctx *ctx, const int32_t *params, int npara
                              const int64_t *dims_l, const float *lhs, int rank_r, const int64_t *dims_r,
                              const float *rhs, float *out, int precision, bbts_stream_t stream);
 
+/* ---- cutensor elementwise / reduction family (libraries/cutensor.cc:26-29) ---- */
+/* Parameters are the reference's int32 words (floats as their bits); blocks use the cutensor
+ * format, column-major payloads; each block form writes the out meta first.  Fewer than 2^31
+ * elements per block.
+ * ew (ew.h:21-41 parse / set_out_meta, 128-186 cpu_op): [f, (c if f == 6), alpha, ordering...];
+ * out = alpha * f(in), out mode r = input mode ordering[r]; f: 0 sigmoid, 1 exp, 2 square,
+ * 3 relu, 4 relu' (v < 0 ? 0 : 1), 5 sqrt, 6 v + c. */
+int bbts_hip_cu_ew(bbts_hip_ctx *ctx, const bbts_tensor_t *in, bbts_tensor_t *out, const int32_t *params, int nparams,
+                   bbts_stream_t stream);
+int bbts_hip_cu_ew_raw(bbts_hip_ctx *ctx, const int32_t *params, int nparams, int rank, const int64_t *dims,
+                       const float *in, float *out, bbts_stream_t stream);
+/* ewb (ewb.h:21-53 parse / set_out_meta, 212-330 cpu_op): [op, alpha, beta, rank_l, ordering_l...,
+ * rank_r, ordering_r...]; out = op(alpha lhs, beta rhs), input mode x is out mode ordering[x],
+ * an out mode missing from an input broadcasts it; op: 0 add, 1 max, 2 min, 3 mul, 4 sub, 5 div. */
+int bbts_hip_cu_ewb(bbts_hip_ctx *ctx, const bbts_tensor_t *lhs, const bbts_tensor_t *rhs, bbts_tensor_t *out,
+                    const int32_t *params, int nparams, bbts_stream_t stream);
+int bbts_hip_cu_ewb_raw(bbts_hip_ctx *ctx, const int32_t *params, int nparams, int rank_l, const int64_t *dims_l,
+                        const float *lhs, int rank_r, const int64_t *dims_r, const float *rhs, float *out,
+                        bbts_stream_t stream);
+/* ewb_castable (ewb_castable.h:23-47): [op]; out = op(lhs, rhs) on one shape, op: 0 add, 1 max,
+ * 2 min, 3 mul (utils.h:37-67); out may alias an input (inputInplace {0, 1}). */
+int bbts_hip_cu_ewb_castable(bbts_hip_ctx *ctx, const bbts_tensor_t *lhs, const bbts_tensor_t *rhs,
+                             bbts_tensor_t *out, const int32_t *params, int nparams, bbts_stream_t stream);
+int bbts_hip_cu_ewb_castable_raw(bbts_hip_ctx *ctx, int op, const float *lhs, const float *rhs, float *out, size_t n,
+                                 bbts_stream_t stream);
+/* reduction (reduction.h:15-37 parse / set_out_meta, 114-177 cpu_op): [op, alpha, ordering...];
+ * out = alpha * fold(op) over the input modes absent from ordering, out mode r = input mode
+ * ordering[r].  When the input's contiguous mode is kept and no split is taken, the fold is
+ * the reference's left fold in column-major order, bit for bit.  segments: 0 picks the split
+ * for occupancy; k > 0 forces k contiguous segments folded separately, then folded in order. */
+int bbts_hip_cu_reduction(bbts_hip_ctx *ctx, const bbts_tensor_t *in, bbts_tensor_t *out, const int32_t *params,
+                          int nparams, bbts_stream_t stream);
+int bbts_hip_cu_reduction_raw(bbts_hip_ctx *ctx, const int32_t *params, int nparams, int rank, const int64_t *dims,
+                              const float *in, float *out, int segments, bbts_stream_t stream);
+
 /* ---- raw-pointer kernels (the kernels behind the block entry points) ----- */
 /* C[m*ldc+n] = alpha * sum_k op(A)[m,k] op(B)[k,n]; row-major; ta: A stored K x M (lda >= M);
  * tb: B stored N x K (ldb >= K).  Batched over `batch` with element strides sa/sb/sc. */

@@ -681,3 +681,168 @@ void orc_gemm_fp8(const uint8_t *QA, const uint8_t *QB, int M, int N, int K, int
         }
     }
 }
+
+/* ---- cutensor elementwise / reduction family ---------------------------------
+ * Column-major blocks of rank <= 4.  Index helpers: the column-major offset of a
+ * coordinate vector, and the coordinates of a linear index. */
+static void orc_coords(int64_t i, int rank, const int64_t *dims, int64_t *c) {
+  for (int r = 0; r < rank; ++r) {
+    c[r] = dims[r] ? i % dims[r] : 0;
+    i = dims[r] ? i / dims[r] : 0;
+  }
+}
+
+/* ew.h:140-171 cpu_op elementwise part: vsExp / vsSqr / vsFmaxI / the relu' loop / vsSqrt /
+ * vsAddI, then cblas_sscal by alpha.  Sigmoid: exp(v) / (1.0 + exp(v)) with the division in
+ * double (ew.h:142-146); where exp(v) overflows FP32 the value is 1, the limit the gpu_op's
+ * cuTENSOR sigmoid gives (the CPU form divides inf by inf there). */
+static float orc_ew_fn(int f, float v, float c) {
+  switch (f) {
+    case 0: {
+      float e = expf(v);
+      return isinf(e) ? 1.0f : (float)((double)e / (1.0 + (double)e));
+    }
+    case 1: return expf(v);
+    case 2: return v * v;
+    case 3: return fmaxf(v, 0.0f);
+    case 4: return v < 0.0f ? 0.0f : 1.0f;
+    case 5: return sqrtf(v);
+    default: return v + c;
+  }
+}
+
+/* ew: out = alpha * f(in); out mode r is input mode ordering[r] and out dims[r] =
+ * dims[ordering[r]] (set_out_meta, ew.h:31-41; the gpu_op's cutensorPermutation with input
+ * modes 0..rank-1 and output modes `ordering`, ew.h:249-258).  The cpu_op's in-place
+ * permute_t lays the same data out for every ordering that is its own inverse. */
+int orc_cu_ew(int f, float c, float alpha, int rank, const int64_t *dims, const int32_t *ord, const float *in,
+              float *out) {
+  if (rank < 0 || rank > ORC_MAXRANK || f < 0 || f > 6) return -1;
+  int64_t od[ORC_MAXRANK], is[ORC_MAXRANK], st = 1, n = 1, co[ORC_MAXRANK];
+  int64_t ist[ORC_MAXRANK];
+  for (int i = 0; i < rank; ++i) {
+    ist[i] = st;
+    st *= dims[i];
+  }
+  for (int r = 0; r < rank; ++r) {
+    od[r] = dims[ord[r]];
+    is[r] = ist[ord[r]];
+    n *= od[r];
+  }
+  for (int64_t o = 0; o < n; ++o) {
+    orc_coords(o, rank, od, co);
+    int64_t off = 0;
+    for (int r = 0; r < rank; ++r) off += co[r] * is[r];
+    out[o] = orc_ew_fn(f, in[off], c) * alpha;
+  }
+  return 0;
+}
+
+/* ewb.h:246-253: vsAddI / vsFmaxI / vsFminI / vsMulI / vsSubI / vsDivI */
+static float orc_bin(int op, float a, float b) {
+  switch (op) {
+    case 0: return a + b;
+    case 1: return fmaxf(a, b);
+    case 2: return fminf(a, b);
+    case 3: return a * b;
+    case 4: return a - b;
+    default: return a / b;
+  }
+}
+
+/* ewb: out[idx] = op(alpha * lhs[idx o ordering_l], beta * rhs[idx o ordering_r]) over the
+ * output's column-major index; an output mode missing from an operand broadcasts it
+ * (ewb.h:228-329: permute_t each operand to ascending labels, scale it, then the strided
+ * vec_op walk of stride_increment_t, ewb.h:55-114).  Output dims come from set_out_meta
+ * (ewb.h:38-53), computed by the caller. */
+int orc_cu_ewb(int op, float alpha, float beta, int rank_l, const int64_t *dl, const int32_t *ol, const float *l,
+               int rank_r, const int64_t *dr, const int32_t *orr, const float *r, int rank_o, const int64_t *dout,
+               float *out) {
+  if (rank_l > ORC_MAXRANK || rank_r > ORC_MAXRANK || rank_o > ORC_MAXRANK || op < 0 || op > 5) return -1;
+  int64_t ls[ORC_MAXRANK] = {0, 0, 0, 0}, rs[ORC_MAXRANK] = {0, 0, 0, 0}, st = 1, n = 1, co[ORC_MAXRANK];
+  for (int x = 0; x < rank_l; ++x) {
+    ls[ol[x]] = st;
+    st *= dl[x];
+  }
+  st = 1;
+  for (int x = 0; x < rank_r; ++x) {
+    rs[orr[x]] = st;
+    st *= dr[x];
+  }
+  for (int k = 0; k < rank_o; ++k) n *= dout[k];
+  for (int64_t o = 0; o < n; ++o) {
+    orc_coords(o, rank_o, dout, co);
+    int64_t lo = 0, ro = 0;
+    for (int k = 0; k < rank_o; ++k) {
+      lo += co[k] * ls[k];
+      ro += co[k] * rs[k];
+    }
+    out[o] = orc_bin(op, alpha * l[lo], beta * r[ro]);
+  }
+  return 0;
+}
+
+/* ewb_castable.h:40-46 cpu_op: op.mkl_op(n, lhs, rhs, out), op from castable_op_t
+ * (utils.h:37-67): vsAdd, vsFmax, vsFmin, vsMul */
+int orc_cu_ewb_castable(int op, int64_t n, const float *l, const float *r, float *out) {
+  if (op < 0 || op > 3) return -1;
+  for (int64_t i = 0; i < n; ++i) out[i] = orc_bin(op, l[i], r[i]);
+  return 0;
+}
+
+/* reduction.h:114-171 cpu_op: the input is laid out as (aggregated modes in input order,
+ * then the output modes in `ordering` order), each output is folded left to right over its
+ * aggregated elements with scalarAdd / scalarMax / scalarMin / scalarMul (utils.h:33-36),
+ * then cblas_sscal by alpha.  Out mode r is input mode ordering[r]. */
+static float orc_red(int op, float a, float b) {
+  switch (op) {
+    case 0: return a + b;
+    case 1: return a > b ? a : b;
+    case 2: return a < b ? a : b;
+    default: return a * b;
+  }
+}
+
+int orc_cu_reduction(int op, float alpha, int rank, const int64_t *dims, const int32_t *ord, int nord,
+                     const float *in, float *out) {
+  if (rank < 0 || rank > ORC_MAXRANK || nord > rank || op < 0 || op > 3) return -1;
+  int kept[ORC_MAXRANK] = {0, 0, 0, 0};
+  int64_t ist[ORC_MAXRANK], st = 1;
+  for (int i = 0; i < rank; ++i) {
+    ist[i] = st;
+    st *= dims[i];
+  }
+  int64_t od[ORC_MAXRANK], ois[ORC_MAXRANK], gd[ORC_MAXRANK], gis[ORC_MAXRANK];
+  int64_t n_out = 1, n_agg = 1;
+  for (int r = 0; r < nord; ++r) {
+    if (ord[r] < 0 || ord[r] >= rank || kept[ord[r]]) return -1;
+    kept[ord[r]] = 1;
+    od[r] = dims[ord[r]];
+    ois[r] = ist[ord[r]];
+    n_out *= od[r];
+  }
+  int ng = 0;
+  for (int i = 0; i < rank; ++i)
+    if (!kept[i]) {
+      gd[ng] = dims[i];
+      gis[ng] = ist[i];
+      n_agg *= dims[i];
+      ++ng;
+    }
+  if (n_out > 0 && n_agg == 0) return -1;
+  int64_t co[ORC_MAXRANK], cg[ORC_MAXRANK];
+  for (int64_t o = 0; o < n_out; ++o) {
+    orc_coords(o, nord, od, co);
+    int64_t base = 0;
+    for (int r = 0; r < nord; ++r) base += co[r] * ois[r];
+    float acc = 0.0f;
+    for (int64_t j = 0; j < n_agg; ++j) {
+      orc_coords(j, ng, gd, cg);
+      int64_t off = base;
+      for (int g = 0; g < ng; ++g) off += cg[g] * gis[g];
+      acc = j == 0 ? in[off] : orc_red(op, acc, in[off]);
+    }
+    out[o] = acc * alpha;
+  }
+  return 0;
+}

@@ -17,6 +17,16 @@ Pinning status (what checks each oracle function; see tests/test_oracle.py):
   contraction plan path ..... brute force einsum (contraction.h:131-202) and numpy.einsum on
                               every plan-consistent mode order; GEMM special cases
   permute ................... numpy.transpose on every permutation of rank <= 4
+  cutensor ew ............... numpy transpose + closed forms per op (sigmoid symmetry and
+                              fixed points, exp(log 2) = 2, exact squares / roots); equal to the
+                              cpu_op's permute_t layout on every self-inverse ordering
+  cutensor ewb .............. numpy broadcasting over label-placed axes; the cpu_op's own
+                              algorithm (permute_t, scale, stride_increment_t walk) restated
+                              in Python on small cases, bit for bit
+  cutensor ewb_castable ..... numpy add / fmax / fmin / multiply, bit for bit
+  cutensor reduction ........ the cpu_op's algorithm (label permutation through permute_t,
+                              then a left fold) restated in Python, bit for bit; numpy
+                              sum / max / min / prod in float64 on larger cases
   rng ....................... moments; the counter mix against its published test values
 Nothing is "parity unpinned".
 """
@@ -285,6 +295,80 @@ def permute(buf, dims, ordering):
     o = np.array(list(ordering) + [0] * (4 - len(ordering)), dtype=np.int32)
     lib().orc_permute_raw(_p(d, i64p), _p(o, i32p), len(dims), _p(b, f32p), _p(tmp, f32p))
     return b
+
+
+# ---- cutensor elementwise / reduction family (libraries/cutensor/{ew,ewb,ewb_castable,reduction}.h)
+def _d4(dims):
+    return np.array(list(dims) + [0] * (4 - len(dims)), dtype=np.int64)
+
+
+def _o4(ordering):
+    return np.array(list(ordering) + [0] * (4 - len(ordering)), dtype=np.int32)
+
+
+def cu_ew(x, dims, f, alpha, ordering, c=0.0):
+    """ew: (out, out_dims), out = alpha * f(x), out mode r = input mode ordering[r]."""
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1)
+    od = tuple(int(dims[m]) for m in ordering)
+    out = np.empty(int(np.prod(od, dtype=np.int64)), dtype=np.float32)
+    rc = lib().orc_cu_ew(int(f), ctypes.c_float(c), ctypes.c_float(alpha), len(dims), _p(_d4(dims), i64p),
+                         _p(_o4(ordering), i32p), _p(x, f32p), _p(out, f32p))
+    if rc != 0:
+        raise ValueError("ew failed")
+    return out, od
+
+
+def cu_ewb_dims(dims_l, ord_l, dims_r, ord_r):
+    """set_out_meta (ewb.h:38-53): the output rank is 1 + the largest label; each output mode
+    takes the extent of the input mode labelled with it."""
+    if not ord_l and not ord_r:
+        return ()
+    ro = 1 + max(list(ord_l) + list(ord_r))
+    d = [None] * ro
+    for x, m in enumerate(ord_l):
+        d[m] = int(dims_l[x])
+    for x, m in enumerate(ord_r):
+        if d[m] is not None and d[m] != int(dims_r[x]):
+            raise ValueError("output mode %d has two extents" % m)
+        d[m] = int(dims_r[x])
+    if any(v is None for v in d):
+        raise ValueError("an output mode is in neither input")
+    return tuple(d)
+
+
+def cu_ewb(lhs, dims_l, ord_l, rhs, dims_r, ord_r, op, alpha=1.0, beta=1.0):
+    """ewb: (out, out_dims), out = op(alpha lhs, beta rhs) with modes matched by label."""
+    od = cu_ewb_dims(dims_l, ord_l, dims_r, ord_r)
+    lhs = np.ascontiguousarray(lhs, dtype=np.float32).reshape(-1)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float32).reshape(-1)
+    out = np.empty(int(np.prod(od, dtype=np.int64)), dtype=np.float32)
+    rc = lib().orc_cu_ewb(int(op), ctypes.c_float(alpha), ctypes.c_float(beta), len(dims_l), _p(_d4(dims_l), i64p),
+                          _p(_o4(ord_l), i32p), _p(lhs, f32p), len(dims_r), _p(_d4(dims_r), i64p),
+                          _p(_o4(ord_r), i32p), _p(rhs, f32p), len(od), _p(_d4(od), i64p), _p(out, f32p))
+    if rc != 0:
+        raise ValueError("ewb failed")
+    return out, od
+
+
+def cu_ewb_castable(op, lhs, rhs):
+    lhs = np.ascontiguousarray(lhs, dtype=np.float32).reshape(-1)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float32).reshape(-1)
+    out = np.empty_like(lhs)
+    if lib().orc_cu_ewb_castable(int(op), lhs.size, _p(lhs, f32p), _p(rhs, f32p), _p(out, f32p)) != 0:
+        raise ValueError("ewb_castable failed")
+    return out
+
+
+def cu_reduction(x, dims, op, alpha, ordering):
+    """reduction: (out, out_dims), the left fold of reduction.h's cpu_op, then times alpha."""
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1)
+    od = tuple(int(dims[m]) for m in ordering)
+    out = np.empty(int(np.prod(od, dtype=np.int64)), dtype=np.float32)
+    rc = lib().orc_cu_reduction(int(op), ctypes.c_float(alpha), len(dims), _p(_d4(dims), i64p),
+                                _p(_o4(ordering), i32p), len(ordering), _p(x, f32p), _p(out, f32p))
+    if rc != 0:
+        raise ValueError("reduction failed")
+    return out, od
 
 
 # ---- FP8 e4m3 forms (this build's definition; the reference's matmul is FP32 only)

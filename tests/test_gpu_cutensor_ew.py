@@ -1,0 +1,297 @@
+"""The cutensor elementwise / reduction family on the GPU through the C ABI, against the
+oracle (libraries/cutensor/{ew, ewb, ewb_castable, reduction}.h semantics).
+
+Bar:
+  * ew: bit-exact for square, relu, relu', sqrt and add-constant; exp and sigmoid within
+    2 ulp of the oracle (both call an FP32 exp; the reference's vsExp is not bit-specified),
+    with an absolute floor of FLT_MIN for results below the normal range;
+  * ewb, ewb_castable: bit-exact (IEEE add / fmax / fmin / mul / sub / div of the scaled
+    operands, products not fused);
+  * reduction: bit-exact with the reference's left fold when the input's contiguous mode is
+    kept and the fold is one segment; max / min bit-exact in every form; add / mul in split or
+    lane-parallel forms within 1e-5 of the FP64 value relative to the sum of magnitudes
+    (add) or within 4 A 2^-24 relative (mul, A folded elements);
+  * block forms: the out meta (rank, dims) byte-exact, parameter errors refused.
+"""
+import itertools
+import struct
+
+import numpy as np
+import pytest
+
+from bbts_amd.device import ew_params, ewb_params, reduction_params
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(torch, x, pad=0):
+    t = torch.zeros(x.size + pad, dtype=torch.float32, device="cuda")
+    t[pad:] = torch.from_numpy(np.ascontiguousarray(x, np.float32))
+    return t[pad:] if pad else t
+
+
+def _out(torch, n, pad=0):
+    t = torch.full((max(n, 1) + pad,), float("nan"), dtype=torch.float32, device="cuda")
+    return t[pad:] if pad else t
+
+
+def _data(n, seed):
+    return np.random.default_rng(seed).standard_normal(n).astype(np.float32)
+
+
+FLT_MIN = float(np.finfo(np.float32).tiny)
+
+
+def _assert_close_ulp(got, ref, ulps, x):
+    """|got - ref| <= ulps ulp(ref), with an absolute floor of FLT_MIN: results below the
+    normal range (exp of inputs under -87.3) may be flushed or rounded at subnormal spacing."""
+    got, ref = np.asarray(got, np.float32), np.asarray(ref, np.float32)
+    both_nan = np.isnan(got) & np.isnan(ref)
+    diff = np.abs(got.astype(np.float64) - ref.astype(np.float64))
+    tol = ulps * np.spacing(np.abs(ref)).astype(np.float64) + FLT_MIN
+    bad = ~(both_nan | (diff <= tol) | (got == ref))
+    idx = np.flatnonzero(bad)[:5]
+    assert not bad.any(), ["x=%r got=%r ref=%r" % (float(x[i]), float(got[i]), float(ref[i])) for i in idx]
+
+
+def _ew(ctx, torch, x, dims, f, alpha, ordering, c=0.0, pad=0):
+    n = int(np.prod(dims)) if dims else 1
+    tin, tout = _dev(torch, x, pad), _out(torch, n, pad)
+    ctx.cu_ew_raw(ew_params(f, alpha, ordering, c), dims, tin, tout)
+    ctx.sync()
+    return tout[:n].cpu().numpy()
+
+
+# ---------------------------------------------------------------- ew
+EW_SHAPES = [(67,), (67, 130), (5, 66, 7), (3, 5, 70, 2), (1, 64, 1, 33), (130, 1, 67)]
+
+
+@pytest.mark.parametrize("dims", EW_SHAPES)
+def test_ew_every_ordering(ctx, orc, torch_cuda, dims):
+    print("ew orderings", dims, flush=True)
+    x = _data(int(np.prod(dims)), len(dims))
+    for ordering in itertools.permutations(range(len(dims))):
+        got = _ew(ctx, torch_cuda, x, dims, 2, 0.75, ordering)
+        ref, _ = orc.cu_ew(x, dims, 2, 0.75, ordering)
+        np.testing.assert_array_equal(got, ref, str(ordering))
+
+
+@pytest.mark.parametrize("f", [0, 1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("dims,ordering", [((1001,), (0,)), ((67, 130), (1, 0)), ((6, 40, 9), (2, 0, 1))])
+def test_ew_each_function(ctx, orc, torch_cuda, f, dims, ordering):
+    x = _data(int(np.prod(dims)), 10 + f) * np.float32(4.0)
+    x[:7] = [0.0, -0.0, 89.0, -89.0, 100.0, 1e-30, -1e-30]
+    got = _ew(ctx, torch_cuda, x, dims, f, -1.25, ordering, c=0.375)
+    ref, _ = orc.cu_ew(x, dims, f, -1.25, ordering, c=0.375)
+    if f in (0, 1):
+        xs, _ = orc.cu_ew(x, dims, 6, 1.0, ordering, c=0.0)  # the inputs in output order
+        _assert_close_ulp(got, ref, 2, xs)
+    else:
+        np.testing.assert_array_equal(got, ref)
+
+
+def test_ew_unaligned_and_large_transposes(ctx, orc, torch_cuda):
+    x = _data(1000003, 1)
+    got = _ew(ctx, torch_cuda, x, (1000003,), 3, 2.0, (0,), pad=1)  # 4-byte aligned only
+    np.testing.assert_array_equal(got, orc.cu_ew(x, (1000003,), 3, 2.0, (0,))[0])
+    for dims, ordering in [((2048, 2048), (1, 0)), ((64, 48, 40, 30), (2, 3, 0, 1)), ((64, 48, 40, 30), (3, 0, 2, 1))]:
+        print("ew transpose", dims, ordering, flush=True)
+        x = _data(int(np.prod(dims)), 2)
+        got = _ew(ctx, torch_cuda, x, dims, 3, 1.0, ordering)
+        np.testing.assert_array_equal(got, orc.cu_ew(x, dims, 3, 1.0, ordering)[0])
+
+
+def test_ew_scalar_and_empty(ctx, orc, torch_cuda):
+    got = _ew(ctx, torch_cuda, np.float32([9.0]), (), 5, 2.0, ())
+    assert got[0] == 6.0
+    tin, tout = _dev(torch_cuda, np.zeros(1, np.float32)), _out(torch_cuda, 1)
+    ctx.cu_ew_raw(ew_params(1, 1.0, (1, 0)), (0, 5), tin, tout)  # no elements: nothing written
+    ctx.sync()
+    assert np.isnan(tout.cpu().numpy()[0])
+
+
+# ---------------------------------------------------------------- ewb
+from test_oracle_cutensor_ew import EWB_CASES  # noqa: E402
+
+EWB_GPU_CASES = EWB_CASES + [
+    ((512, 768), (0, 1), (768,), (1,)),
+    ((512, 768), (1, 0), (512, 768), (1, 0)),
+    ((96, 80, 12), (0, 1, 2), (12, 96), (2, 0)),
+    ((1, 300, 1), (0, 1, 2), (300,), (1,)),
+]
+
+
+@pytest.mark.parametrize("case", EWB_GPU_CASES)
+def test_ewb_every_op(ctx, orc, torch_cuda, case):
+    dl, ol, dr, orr = case
+    print("ewb", case, flush=True)
+    od = orc.cu_ewb_dims(dl, ol, dr, orr)
+    n = int(np.prod(od))
+    l = _data(int(np.prod(dl)), 3)
+    r = _data(int(np.prod(dr)), 4)
+    r[::7] = 0.0  # divisions by zero and signed zeros
+    tl, tr = _dev(torch_cuda, l), _dev(torch_cuda, r)
+    for op in range(6):
+        for alpha, beta in ((1.0, 1.0), (1.5, -0.75)):
+            tout = _out(torch_cuda, n)
+            ctx.cu_ewb_raw(ewb_params(op, alpha, beta, ol, orr), dl, tl, dr, tr, tout)
+            ctx.sync()
+            ref, _ = orc.cu_ewb(l, dl, ol, r, dr, orr, op, alpha, beta)
+            np.testing.assert_array_equal(tout[:n].cpu().numpy(), ref, "op %d" % op)
+
+
+def test_ewb_scalar_and_unaligned(ctx, orc, torch_cuda):
+    tl, tr, tout = _dev(torch_cuda, np.float32([3.0])), _dev(torch_cuda, np.float32([4.0])), _out(torch_cuda, 1)
+    ctx.cu_ewb_raw(ewb_params(5, 2.0, 0.5, (), ()), (), tl, (), tr, tout)
+    ctx.sync()
+    assert tout.cpu().numpy()[0] == 3.0
+    l, r = _data(4099, 5), _data(4099, 6)
+    tl, tr, tout = _dev(torch_cuda, l, 1), _dev(torch_cuda, r, 3), _out(torch_cuda, 4099, 2)
+    ctx.cu_ewb_raw(ewb_params(0, 0.5, 2.0, (0,), (0,)), (4099,), tl, (4099,), tr, tout)
+    ctx.sync()
+    np.testing.assert_array_equal(tout.cpu().numpy(), orc.cu_ewb(l, (4099,), (0,), r, (4099,), (0,), 0, 0.5, 2.0)[0])
+
+
+# ---------------------------------------------------------------- ewb_castable
+@pytest.mark.parametrize("n", [1 << 20, 1000003])
+def test_ewb_castable(ctx, orc, torch_cuda, n):
+    l, r = _data(n, 7), _data(n, 8)
+    l[::1001] = np.nan
+    tl, tr = _dev(torch_cuda, l), _dev(torch_cuda, r)
+    for op in range(4):
+        tout = _out(torch_cuda, n)
+        ctx.cu_ewb_castable_raw(op, tl, tr, tout)
+        ctx.sync()
+        np.testing.assert_array_equal(tout.cpu().numpy(), orc.cu_ewb_castable(op, l, r))
+    # in place: out aliases lhs (inputInplace {0, 1}, ewb_castable.h:99)
+    ctx.cu_ewb_castable_raw(3, tl, tr, tl)
+    ctx.sync()
+    np.testing.assert_array_equal(tl.cpu().numpy(), orc.cu_ewb_castable(3, l, r))
+
+
+# ---------------------------------------------------------------- reduction
+def _red(ctx, torch, x, dims, op, alpha, ordering, segments):
+    n = int(np.prod([dims[m] for m in ordering])) if ordering else 1
+    tin, tout = _dev(torch, x), _out(torch, n)
+    ctx.cu_reduction_raw(reduction_params(op, alpha, ordering), dims, tin, tout, segments)
+    ctx.sync()
+    return tout[:n].cpu().numpy()
+
+
+def _red_check(orc, x, dims, op, alpha, ordering, got, exact):
+    ref, _ = orc.cu_reduction(x, dims, op, alpha, ordering)
+    if exact or op in (1, 2):
+        np.testing.assert_array_equal(got, ref, str(ordering))
+        return
+    X = x.reshape(tuple(dims)[::-1]).transpose().astype(np.float64)
+    agg = tuple(i for i in range(len(dims)) if i not in ordering)
+    A = int(np.prod([dims[a] for a in agg])) if agg else 1
+    perm = list(ordering) + list(agg)
+    shape = [dims[m] for m in ordering]
+    if op == 0:
+        ex = np.transpose(X.sum(axis=agg, keepdims=True), perm).reshape(shape) * alpha
+        mag = np.transpose(np.abs(X).sum(axis=agg, keepdims=True), perm).reshape(shape) * abs(alpha)
+        ex, mag = ex.transpose().reshape(-1), mag.transpose().reshape(-1)
+        assert np.all(np.abs(got - ex) <= 1e-5 * mag + 1e-30), str(ordering)
+    else:
+        ex = np.transpose(X.prod(axis=agg, keepdims=True), perm).reshape(shape).transpose().reshape(-1) * alpha
+        assert np.all(np.abs(got - ex) <= 4 * A * 2.0 ** -24 * np.abs(ex) + 1e-30), str(ordering)
+
+
+def _orderings(rank):
+    for k in range(rank + 1):
+        yield from itertools.permutations(range(rank), k)
+
+
+@pytest.mark.parametrize("dims", [(67, 5, 130), (6, 5, 40, 3), (300, 7)])
+@pytest.mark.parametrize("op", [0, 1, 2, 3])
+def test_reduction_every_ordering(ctx, orc, torch_cuda, dims, op):
+    print("reduction", dims, op, flush=True)
+    x = _data(int(np.prod(dims)), 20 + op)
+    if op == 3:
+        x = np.float32(1.0) + np.float32(0.01) * x
+    for ordering in _orderings(len(dims)):
+        keeps_mode0 = 0 in ordering or all(dims[i] == 1 for i in range(len(dims)) if i not in ordering)
+        for segments in (1, 0, 3):
+            got = _red(ctx, torch_cuda, x, dims, op, 1.5, ordering, segments)
+            _red_check(orc, x, dims, op, 1.5, ordering, got, exact=keeps_mode0 and segments == 1)
+
+
+@pytest.mark.parametrize("dims,ordering", [((2048, 2048), (0,)), ((2048, 2048), (1,)), ((2048, 2048), ()),
+                                           ((96, 96, 128), (2, 0)), ((96, 96, 128), (1,)), ((16, 2048, 64), (0, 2))])
+def test_reduction_large(ctx, orc, torch_cuda, dims, ordering):
+    print("reduction large", dims, ordering, flush=True)
+    x = _data(int(np.prod(dims)), 5)
+    for op in (0, 1):
+        got = _red(ctx, torch_cuda, x, dims, op, 0.5, ordering, 0)
+        _red_check(orc, x, dims, op, 0.5, ordering, got, exact=False)
+    if 0 in ordering:  # one segment: the reference's left fold, bit for bit
+        got = _red(ctx, torch_cuda, x, dims, 0, 0.5, ordering, 1)
+        _red_check(orc, x, dims, 0, 0.5, ordering, got, exact=True)
+
+
+# ---------------------------------------------------------------- block forms
+def test_block_forms_meta_and_values(ctx, orc, torch_cuda):
+    dims = (40, 30, 6)
+    x = _data(7200, 9)
+    blk = ctx.cutensor(dims, x)
+    # ew with a reordering: out meta dims[ordering[r]]
+    out = ctx.alloc(orc.cu_header([1]).tobytes(), 260 + 4 * 7200)
+    ctx.cu_ew(blk, out, ew_params(3, 2.0, (2, 0, 1)))
+    ctx.sync()
+    assert out.cu_shape() == (6, 40, 30)
+    hdr = out.download_bytes(260).tobytes()
+    assert struct.unpack_from("<iI", hdr, 0) == (1, 3) and struct.unpack_from("<3q", hdr, 12) == (6, 40, 30)
+    np.testing.assert_array_equal(out.download_payload(7200), orc.cu_ew(x, dims, 3, 2.0, (2, 0, 1))[0])
+    # reduction to (6, 40)
+    red = ctx.alloc(orc.cu_header([1]).tobytes(), 260 + 4 * 240)
+    ctx.cu_reduction(blk, red, reduction_params(0, 1.0, (2, 0)))
+    ctx.sync()
+    assert red.cu_shape() == (6, 40)
+    got = red.download_payload(240)
+    _red_check(orc, x, dims, 0, 1.0, (2, 0), got, exact=False)
+    # ewb: x + bias over mode 1
+    b = _data(30, 10)
+    bias = ctx.cutensor((30,), b)
+    o2 = ctx.alloc(orc.cu_header([1]).tobytes(), 260 + 4 * 7200)
+    ctx.cu_ewb(blk, bias, o2, ewb_params(0, 1.0, 1.0, (0, 1, 2), (1,)))
+    ctx.sync()
+    assert o2.cu_shape() == dims
+    np.testing.assert_array_equal(o2.download_payload(7200), orc.cu_ewb(x, dims, (0, 1, 2), b, (30,), (1,), 0)[0])
+    # ewb_castable in place on lhs
+    y = _data(7200, 11)
+    blk2 = ctx.cutensor(dims, y)
+    ctx.cu_ewb_castable(blk, blk2, blk, 1)
+    ctx.sync()
+    np.testing.assert_array_equal(blk.download_payload(7200), orc.cu_ewb_castable(1, x, y))
+    assert blk.cu_shape() == dims
+    for t in (blk, out, red, bias, o2, blk2):
+        t.free()
+
+
+def test_parameter_errors(ctx, orc, torch_cuda):
+    from bbts_amd import BBTSError
+    tin, tout = _dev(torch_cuda, np.zeros(64, np.float32)), _out(torch_cuda, 64)
+    bad = [
+        lambda: ctx.cu_ew_raw(ew_params(7, 1.0, (0, 1)), (8, 8), tin, tout),        # not a unary op
+        lambda: ctx.cu_ew_raw(ew_params(1, 1.0, (0,)), (8, 8), tin, tout),          # ordering of the wrong rank
+        lambda: ctx.cu_ew_raw(ew_params(1, 1.0, (1, 1)), (8, 8), tin, tout),        # repeated mode
+        lambda: ctx.cu_ew_raw(ew_params(1, 1.0, (1, 0)), (8, 8), tin, tin),         # reordering in place
+        lambda: ctx.cu_ewb_raw(ewb_params(6, 1.0, 1.0, (0,), (0,)), (8,), tin, (8,), tin, tout),   # not an op
+        lambda: ctx.cu_ewb_raw(ewb_params(0, 1.0, 1.0, (0,), (0,)), (8,), tin, (9,), tin, tout),   # two extents
+        lambda: ctx.cu_ewb_raw(ewb_params(0, 1.0, 1.0, (0,), (2,)), (8,), tin, (8,), tin, tout),   # mode 1 unnamed
+        lambda: ctx.cu_ewb_castable_raw(4, tin, tin, tout),                                         # not castable
+        lambda: ctx.cu_reduction_raw(reduction_params(4, 1.0, (0,)), (8, 8), tin, tout),          # not castable
+        lambda: ctx.cu_reduction_raw(reduction_params(0, 1.0, (2,)), (8, 8), tin, tout),          # rank too small
+        lambda: ctx.cu_reduction_raw(reduction_params(0, 1.0, (1, 1)), (8, 8), tin, tout),        # repeated mode
+    ]
+    for i, fn in enumerate(bad):
+        with pytest.raises(BBTSError):
+            fn()
+        print("refused", i, flush=True)
+    small = ctx.alloc(orc.cu_header([1]).tobytes(), 260 + 4 * 10)
+    blk = ctx.cutensor((8, 8), np.zeros(64, np.float32))
+    with pytest.raises(BBTSError):
+        ctx.cu_ew(blk, small, ew_params(3, 1.0, (1, 0)))  # output block too small
+    small.free()
+    blk.free()

@@ -132,6 +132,31 @@ def test_contraction_command(rt, orc):
     assert np.linalg.norm(got - ref) <= 2e-6 * np.linalg.norm(ref)
 
 
+def test_cutensor_family_commands(rt, orc):
+    # libraries/cutensor.cc:25-29 order after contraction
+    from bbts_amd.device import ew_params, ewb_params, reduction_params
+    assert [rt.impl(n, True) for n in ("reduction", "ew", "ewb", "ewb_castable")] == [(5, 0), (6, 0), (7, 0), (8, 0)]
+    dims = [30, 20, 4]
+    X = orc.normal(2400, 3)
+    B = orc.normal(20, 4)
+    rt.put_cu(1, dims, X)
+    rt.put_cu(2, [20], B)
+    rt.apply(rt.impl("ewb", True), [1, 2], [3], ewb_params(0, 1.0, -1.0, (0, 1, 2), (1,)))  # X - B along mode 1
+    rt.apply(rt.impl("ew", True), [3], [4], ew_params(3, 2.0, (2, 0, 1)))  # 2 relu, modes (4, 30, 20)
+    rt.apply(rt.impl("reduction", True), [4], [5], reduction_params(1, 1.0, (2, 0)))  # max over the 30: (20, 4)
+    rt.apply(rt.impl("ewb_castable", True), [5, 5], [6], [3])  # square through mul
+    rt.run()
+    y3, d3 = orc.cu_ewb(X, dims, (0, 1, 2), B, (20,), (1,), 0, 1.0, -1.0)
+    y4, d4 = orc.cu_ew(y3, d3, 3, 2.0, (2, 0, 1))
+    y5, d5 = orc.cu_reduction(y4, d4, 1, 1.0, (2, 0))
+    y6 = orc.cu_ewb_castable(3, y5, y5)
+    for tid, d, y in ((3, d3, y3), (4, d4, y4), (5, d5, y5), (6, d5, y6)):
+        meta, _ = rt.meta(tid)
+        assert struct.unpack_from("<iI", meta, 0) == (1, len(d))
+        assert struct.unpack_from("<%dq" % len(d), meta, 12) == tuple(d)
+        assert np.array_equal(rt.get_bytes(tid)[260:].view(np.float32)[:len(y)], y), tid
+
+
 def test_command_errors(rt):
     from bbts_amd import BBTSError
     from bbts_amd.runtime import rtlib
