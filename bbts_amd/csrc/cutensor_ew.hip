@@ -151,7 +151,9 @@ struct transpose_t {
   uint32_t is0, os0, is1, os1;  // batch strides
 };
 
-template <int F>
+// V4: A and B are multiples of 4 and both buffers 16-byte aligned (every stride is then a
+// multiple of 4 too), so global reads and writes move float4s; the LDS tile stays scalar.
+template <int F, bool V4>
 __global__ __launch_bounds__(kT) void ew_transpose_kernel(const float *__restrict__ in, float *__restrict__ out,
                                                          transpose_t t, float alpha, float c) {
   __shared__ float tile[64][65];
@@ -166,6 +168,35 @@ __global__ __launch_bounds__(kT) void ew_transpose_kernel(const float *__restric
   const uint32_t c0 = id - q * t.e0.d, c1 = q;
   const float *src = in + c0 * t.is0 + c1 * t.is1;
   float *dst = out + c0 * t.os0 + c1 * t.os1;
+  if (V4) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int l = threadIdx.x + k * kT, bq = (l & 15) * 4, al = l >> 4;
+      const uint32_t a = a0 + al, b = b0 + bq;
+      if (a < t.A && b < t.B) {
+        const float4 v = *reinterpret_cast<const float4 *>(src + a * t.is_a + b);
+        tile[al][bq] = v.x;
+        tile[al][bq + 1] = v.y;
+        tile[al][bq + 2] = v.z;
+        tile[al][bq + 3] = v.w;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int l = threadIdx.x + k * kT, aq = (l & 15) * 4, bl = l >> 4;
+      const uint32_t a = a0 + aq, b = b0 + bl;
+      if (a < t.A && b < t.B) {
+        float4 o;
+        o.x = ew_fn<F>(tile[aq][bl], c) * alpha;
+        o.y = ew_fn<F>(tile[aq + 1][bl], c) * alpha;
+        o.z = ew_fn<F>(tile[aq + 2][bl], c) * alpha;
+        o.w = ew_fn<F>(tile[aq + 3][bl], c) * alpha;
+        *reinterpret_cast<float4 *>(dst + a + b * t.os_b) = o;
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     const int l = threadIdx.x + k * kT, bl = l & 63, al = l >> 6;
@@ -280,6 +311,7 @@ struct red_t {
   uint32_t ge[2] = {1, 1}, gs[2] = {0, 0};  // aggregated extents and strides (<= 2 after merging)
   fastdiv g0;            // ge[0] as a divisor
   uint32_t num_out = 1, A = 1, len = 1, S = 1;  // outputs, fold length, segment length, segments
+  int vec = 0;           // wave form: float4 reads of the contiguous run
 };
 
 // Input mode 0 kept: one thread per (output, segment), lanes along input mode 0 (coalesced
@@ -349,7 +381,12 @@ __global__ __launch_bounds__(kT) void red_wave_kernel(const float *__restrict__ 
   const float *p = in + base;
   const uint32_t f0 = seg * g.len, f1 = min(g.A, f0 + g.len);
   float acc = red_identity<OP>();
-  if (g.ng == 1) {
+  if (g.ng == 1 && g.vec) {  // run length and base offsets multiples of 4, input 16-byte aligned
+    for (uint32_t f = f0 + 4 * lane; f < f1; f += 256) {
+      const float4 v = *reinterpret_cast<const float4 *>(p + f);
+      acc = red_op<OP>(red_op<OP>(red_op<OP>(red_op<OP>(acc, v.x), v.y), v.z), v.w);
+    }
+  } else if (g.ng == 1) {
     for (uint32_t f = f0 + lane; f < f1; f += 64) acc = red_op<OP>(acc, p[f]);
   } else {
     for (uint32_t f = f0 + lane; f < f1; f += 64) {
@@ -363,6 +400,20 @@ __global__ __launch_bounds__(kT) void red_wave_kernel(const float *__restrict__ 
     if (g.S == 1) dst[o] = acc * alpha;
     else dst[seg * g.num_out + o] = acc;
   }
+}
+
+// out[o] = alpha * fold over segments, one wave per output: lanes stride the segments, then
+// a butterfly combines the lanes
+template <int OP>
+__global__ __launch_bounds__(kT) void red_combine_wave_kernel(const float *__restrict__ part, float *__restrict__ out,
+                                                             uint32_t num_out, uint32_t S, float alpha) {
+  const uint32_t o = blockIdx.x * (kT / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (o >= num_out) return;  // whole waves leave together
+  float acc = red_identity<OP>();
+  for (uint32_t s = lane; s < S; s += 64) acc = red_op<OP>(acc, part[s * num_out + o]);
+#pragma unroll
+  for (int sh = 32; sh > 0; sh >>= 1) acc = red_op<OP>(acc, __shfl_xor(acc, sh));
+  if (lane == 0) out[o] = acc * alpha;
 }
 
 // out[o] = alpha * fold over segments, in segment order
@@ -530,7 +581,10 @@ int launch_ew_f(const float *in, float *out, int rank, const int64_t *dims, cons
   t.os1 = bos[1];
   const uint64_t tiles = (uint64_t)ta * tb * ext[0] * ext[1];
   BBTS_REQUIRE(tiles < (uint64_t(1) << 31), BBTS_E_UNSUPPORTED, "too many transpose tiles");
-  hipLaunchKernelGGL(ew_transpose_kernel<F>, dim3((unsigned)tiles), dim3(kT), 0, s, in, out, t, alpha, c);
+  const bool v4 = t.A % 4 == 0 && t.B % 4 == 0 && reinterpret_cast<uintptr_t>(in) % 16 == 0 &&
+                  reinterpret_cast<uintptr_t>(out) % 16 == 0;
+  if (v4) hipLaunchKernelGGL((ew_transpose_kernel<F, true>), dim3((unsigned)tiles), dim3(kT), 0, s, in, out, t, alpha, c);
+  else hipLaunchKernelGGL((ew_transpose_kernel<F, false>), dim3((unsigned)tiles), dim3(kT), 0, s, in, out, t, alpha, c);
   BBTS_LAUNCH_CHECK();
   return BBTS_OK;
 }
@@ -757,8 +811,12 @@ int launch_red_op(bbts_hip_ctx *ctx, const float *in, float *out, red_t g, bool 
                           dst, g, alpha);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess && g.S > 1) {
-    hipLaunchKernelGGL(red_combine_kernel<OP>, dim3(blocks_for(g.num_out, kT)), dim3(kT), 0, s, dst, out, g.num_out,
-                       g.S, alpha);
+    if (g.S > 16)
+      hipLaunchKernelGGL(red_combine_wave_kernel<OP>, dim3(blocks_exact(g.num_out, kT / 64)), dim3(kT), 0, s, dst,
+                         out, g.num_out, g.S, alpha);
+    else
+      hipLaunchKernelGGL(red_combine_kernel<OP>, dim3(blocks_for(g.num_out, kT)), dim3(kT), 0, s, dst, out,
+                         g.num_out, g.S, alpha);
     e = hipGetLastError();
   }
   if (e != hipSuccess) {
@@ -862,7 +920,7 @@ int run_reduction(bbts_hip_ctx *ctx, const int32_t *w, int nw, int rank, const i
   } else if (wave) {
     if (num_out < 8192 && A > 512) S = std::min<uint64_t>((8192 + num_out - 1) / num_out, (A + 511) / 512);
   } else {
-    if (num_out < 65536 && A > 128) S = std::min<uint64_t>((65536 + num_out - 1) / num_out, (A + 63) / 64);
+    if (num_out < 262144 && A > 32) S = std::min<uint64_t>((262144 + num_out - 1) / num_out, (A + 15) / 16);
   }
   S = std::max<uint64_t>(1, std::min<uint64_t>(S, A));
   uint64_t len = (A + S - 1) / S;
@@ -871,6 +929,7 @@ int run_reduction(bbts_hip_ctx *ctx, const int32_t *w, int nw, int rank, const i
   BBTS_REQUIRE(S * num_out < (uint64_t(1) << 31), BBTS_E_UNSUPPORTED, "too many reduction segments");
   g.len = (uint32_t)len;
   g.S = (uint32_t)S;
+  g.vec = wave && g.ng == 1 && g.ge[0] % 4 == 0 && reinterpret_cast<uintptr_t>(in) % 16 == 0;
   switch (op) {
     case 0: return launch_red_op<0>(ctx, in, out, g, wave, alpha, s);
     case 1: return launch_red_op<1>(ctx, in, out, g, wave, alpha, s);
