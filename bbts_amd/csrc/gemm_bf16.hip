@@ -411,11 +411,23 @@ void choose_configs(bbts_batch *b, const std::vector<float> &cost) {
       pick = best >= 0 ? best : 1;
     b->cfg[i] = pick;
   }
-  // Every config other than the base becomes a launch of its own, which adds a launch gap and
-  // a drain tail (about 25 us on a 256-CU part for a small group). Ops leave the base config
-  // only when their summed saving pays for that.
-  const double kGroupUs = 25.0;
+  // Every config other than the base becomes a launch of its own, stream-ordered after the
+  // base group: it adds a launch gap and cannot start before the base group has drained, which
+  // takes about its longest tile (cost per copy / tiles x resident tiles; a 256 x 256 tile at
+  // K = 4096 runs for over 100 us).  Ops leave the base config only when their summed saving
+  // pays for that.
+  const double kLaunchUs = 10.0;
   if (base >= 0) {
+    double longest = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+      if (b->cfg[i] != base) continue;
+      int variant = 0;
+      variant_of(b->ops[i].dtype, b->prec[i], &variant);
+      const double slots = (double)b->ctx->num_cus * occupancy(base, variant);
+      const double tiles = (double)std::max<long long>(1, tiles_of(b->ops[i], base));
+      longest = std::max(longest, (double)cost[i * kNumCfgs + (size_t)base] / tiles * slots);
+    }
+    const double penalty = kLaunchUs + longest;
     std::vector<double> saving((size_t)kNumCfgs, 0.0);
     for (size_t i = 0; i < n; ++i) {
       const int c = b->cfg[i];
@@ -423,13 +435,15 @@ void choose_configs(bbts_batch *b, const std::vector<float> &cost) {
     }
     for (size_t i = 0; i < n; ++i) {
       const int c = b->cfg[i];
-      if (c != base && saving[(size_t)c] < kGroupUs) b->cfg[i] = base;
+      if (c != base && saving[(size_t)c] < penalty) b->cfg[i] = base;
     }
   }
+  // Within a group every tile has one shape, so its work is proportional to K: the queues are
+  // packed on tile area x K, not on cost / tiles of each op, whose timing noise packed
+  // measurably worse (2-4 % on the 512-op stream with every op on one config).
   for (size_t i = 0; i < n; ++i) {
     const int pick = b->cfg[i];
-    float c = cost[i * kNumCfgs + (size_t)pick];
-    b->tile_us[i] = c > 0 ? c / (float)tiles_of(b->ops[i], pick) : 0.0f;
+    b->tile_us[i] = (float)kCfgs[pick].bm * kCfgs[pick].bn * (float)b->ops[i].K * 1e-9f;
   }
 }
 
@@ -868,14 +882,15 @@ int bbts_hip_tune_batch(bbts_batch *b, int reps, float *times_us) {
     const size_t bytesC = ((size_t)(d.M - 1) * (size_t)d.ldc + (size_t)d.N) * 4;
     auto up = [](size_t x) { return (x + 255) / 256 * 256; };
     const size_t per_copy = up(bytesA) + up(bytesB) + up(bytesC);
+    // each config is timed at R and 2R copies (R fills the chip about four times over)
     long long Rmax = 1;
     for (int c = 0; c < kNumCfgs; ++c) {
       long long tiles = tiles_of(d, c);
       long long R = (4LL * b->ctx->num_cus * occupancy(c, variant) + tiles - 1) / tiles;
-      if (R > Rmax) Rmax = R;
+      if (2 * R > Rmax) Rmax = 2 * R;
     }
-    // at most 256 copies and 1 GiB of copies per shape
-    Rmax = std::min<long long>(Rmax, 256);
+    // at most 512 copies and 1 GiB of copies per shape
+    Rmax = std::min<long long>(Rmax, 512);
     Rmax = std::max<long long>(1, std::min<long long>(Rmax, (long long)((size_t(1) << 30) / std::max<size_t>(per_copy, 1))));
     void *copies = nullptr;
     size_t copies_rounded = 0;
@@ -901,26 +916,41 @@ int bbts_hip_tune_batch(bbts_batch *b, int reps, float *times_us) {
       long long tiles = tiles_of(d, c);
       long long target = 4LL * b->ctx->num_cus * occupancy(c, variant);
       long long R = (target + tiles - 1) / tiles;
-      if (R < 1) R = 1;
-      if (R > Rmax) R = Rmax;
-      tmp.ops.assign(reps_desc.begin(), reps_desc.begin() + R);
-      tmp.prec.assign((size_t)R, prec);
-      tmp.planned = false;
-      if ((rc = bbts_hip_batch_plan(&tmp, BBTS_SCHED_FORCED_CFG0 + c, nullptr))) break;
-      if ((rc = bbts_hip_batch_launch(&tmp, s))) break;  // warm-up
-      hipEventRecord(e0, s);
-      for (int r = 0; r < reps && rc == BBTS_OK; ++r) rc = bbts_hip_batch_launch(&tmp, s);
-      hipEventRecord(e1, s);
-      if (rc) break;
-      hipError_t e = hipEventSynchronize(e1);
-      if (e != hipSuccess) {
-        set_error("tuning launch failed: %s", hipGetErrorString(e));
-        rc = BBTS_E_HIP;
-        break;
+      R = std::max<long long>(1, std::min<long long>(R, Rmax / 2));
+      const long long R2 = std::min<long long>(2 * R, Rmax);
+      // microseconds per launch of n copies
+      auto time_copies = [&](long long ncopy, float *us_per_launch) -> int {
+        tmp.ops.assign(reps_desc.begin(), reps_desc.begin() + ncopy);
+        tmp.prec.assign((size_t)ncopy, prec);
+        tmp.planned = false;
+        int r2 = bbts_hip_batch_plan(&tmp, BBTS_SCHED_FORCED_CFG0 + c, nullptr);
+        if (!r2) r2 = bbts_hip_batch_launch(&tmp, s);  // warm-up
+        if (r2) return r2;
+        hipEventRecord(e0, s);
+        for (int r = 0; r < reps && r2 == BBTS_OK; ++r) r2 = bbts_hip_batch_launch(&tmp, s);
+        hipEventRecord(e1, s);
+        if (r2) return r2;
+        hipError_t e = hipEventSynchronize(e1);
+        if (e != hipSuccess) {
+          set_error("tuning launch failed: %s", hipGetErrorString(e));
+          return BBTS_E_HIP;
+        }
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, e0, e1);
+        *us_per_launch = ms * 1000.0f / (float)reps;
+        return BBTS_OK;
+      };
+      // The marginal cost of a copy, (t(2R) - t(R)) / R, holds no launch overhead and no drain
+      // tail: a grouped launch of a whole stream pays those once, not once per op.  The
+      // difference is floored at half the plain average so that timing noise cannot make an
+      // op look nearly free.
+      float t1 = 0.0f, t2 = 0.0f;
+      if ((rc = time_copies(R, &t1))) break;
+      float us = t1 / (float)R;
+      if (R2 > R) {
+        if ((rc = time_copies(R2, &t2))) break;
+        us = std::max((t2 - t1) / (float)(R2 - R), 0.5f * t2 / (float)R2);
       }
-      float ms = 0.0f;
-      hipEventElapsedTime(&ms, e0, e1);
-      float us = ms * 1000.0f / (float)(reps * R);
       all[(size_t)c] = us;
       if (times_us)
         for (size_t i : kv.second) times_us[i * kNumCfgs + (size_t)c] = us;
