@@ -360,10 +360,190 @@ __device__ __forceinline__ void tile_fast(const op_view &op, int m0, int n0, uin
     }
 }
 
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+// s_waitcnt with vmcnt <= n (n < 64) and the LDS / export counters unconstrained (gfx9 encoding:
+// vmcnt[3:0] in bits 3:0 and vmcnt[5:4] in bits 15:14, expcnt bits 6:4, lgkmcnt bits 11:8)
+template <int n>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(n >= 0 && n < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((n & 15) | (7 << 4) | (15 << 8) | ((n >> 4) << 14));
+}
+
+// BF16 tiles of 256 rows: the operand stages go from HBM straight into LDS (buffer loads with
+// the LDS-destination bit; gfx950 moves 16 bytes per lane, lane l of an instruction writing
+// the 16 bytes at the instruction's LDS base + 16 l).  No register staging and no LDS write
+// instructions: the VGPRs the two prefetch register sets held are free, and the issue slots
+// the LDS writes took go to the MFMA pipe.  Three LDS stages keep two stages of loads in
+// flight; each lane loads the global chunk whose swizzled position is its LDS slot (the XOR
+// swizzles of sw / swn are involutions), so the compute reads are those of tile_fast.
+// Synchronisation per stage: issue stage k+2, compute stage k, wait until at most stage k+2's
+// loads are outstanding (this wave's share of stage k+1 has landed), then a barrier without a
+// fence (every wave's share of stage k+1 has landed and stage k's buffer is free).
+template <int BM, int BN>
+__device__ __forceinline__ void tile_dma(const op_view &op, int m0, int n0) {
+  constexpr int BK = stage_k<BM, BN, 1>::value;
+  static_assert(BK == 32, "three 32-deep stages");
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int FM = WM / 32, FN = WN / 32;
+  constexpr int CPR = BK / 8;          // 16-byte chunks per A row
+  constexpr int CPB = BN / 8;          // 16-byte chunks per B k-row
+  constexpr int AJ = BM * CPR / 256;   // A loads per lane per stage
+  constexpr int BJ = BK * CPB / 256;   // B loads per lane per stage
+  static_assert(AJ * 256 == BM * CPR && BJ * 256 == BK * CPB, "whole instructions per stage");
+  __shared__ __align__(16) uint16_t sA0[BM * BK], sA1[BM * BK], sA2[BM * BK];
+  __shared__ __align__(16) uint16_t sB0[BK * BN], sB1[BK * BN], sB2[BK * BN];
+
+  const int M = op.M, N = op.N, K = op.K;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave >> 1, wn = wave & 1;
+  const __amdgpu_buffer_rsrc_t rA = make_rsrc(op.A, ((int64_t)(M - 1) * op.lda + K) * 2);
+  const __amdgpu_buffer_rsrc_t rB = make_rsrc(op.B, ((int64_t)(K - 1) * op.ldb + N) * 2);
+
+  // lane's chunks: instruction j of wave w fills LDS chunks (4 j + w) * 64 + lane
+  uint32_t va[AJ], vb[BJ];
+  int ka[AJ], kb[BJ];
+#pragma unroll
+  for (int j = 0; j < AJ; ++j) {
+    const int p = (j * 4 + wave) * 64 + lane, row = p / CPR, slot = p % CPR;
+    const int kc = slot ^ ((row >> 2) & 3);  // sw<32>: the logical k chunk stored in this slot
+    const int gm = m0 + row;
+    va[j] = gm < M ? (uint32_t)(((int64_t)gm * op.lda + kc * 8) * 2) : kOOB;
+    ka[j] = kc * 8;
+  }
+#pragma unroll
+  for (int j = 0; j < BJ; ++j) {
+    const int p = (j * 4 + wave) * 64 + lane, krow = p / CPB, slot = p % CPB;
+    const int nc = slot ^ (((krow & 3) << 2) & (CPB - 1));  // swn<BN>
+    const int gn = n0 + nc * 8;
+    vb[j] = gn < N ? (uint32_t)(((int64_t)krow * op.ldb + gn) * 2) : kOOB;
+    kb[j] = krow;
+  }
+  // Stage at k0 into (sa, sb).  Lanes past M, N or K get the out-of-range offset (the
+  // descriptor's range check writes zeros and reads nothing); a stage at or past K reads
+  // nothing and has a zero scalar offset, so no offset can wrap back into range.
+  auto issue = [&](uint16_t *sa, uint16_t *sb, int k0) {
+    const int lim = K - k0;
+    const uint32_t soa = k0 < K ? (uint32_t)k0 * 2u : 0u;
+    const uint32_t sob = k0 < K ? (uint32_t)((int64_t)k0 * op.ldb * 2) : 0u;
+#pragma unroll
+    for (int j = 0; j < AJ; ++j)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rA, (lds_void_t *)(sa + (j * 4 + wave) * 64 * 8), 16,
+                                               ka[j] < lim ? va[j] : kOOB, soa, 0, 0);
+#pragma unroll
+    for (int j = 0; j < BJ; ++j)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rB, (lds_void_t *)(sb + (j * 4 + wave) * 64 * 8), 16,
+                                               kb[j] < lim ? vb[j] : kOOB, sob, 0, 0);
+  };
+
+  f16v acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
+
+  const int half = lane >> 5, col = lane & 31;
+  auto compute = [&](const uint16_t *as, const uint16_t *bs) {
+    __builtin_amdgcn_iglp_opt(0);
+#pragma unroll
+    for (int kk = 0; kk < BK / 16; ++kk) {
+      const int k = kk * 16 + half * 8;
+      bf16x8 af[FM], bfr[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+        af[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&as[sw<BK>(wm * WM + i * 32 + col, k)]));
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int g = lane >> 4, i = lane & 15;
+        const int kr = kk * 16 + 8 * (g >> 1) + (i >> 2);
+        const int nc = wn * WN + j * 32 + 16 * (g & 1) + 4 * (i & 3);
+        struct {
+          v4s a, b;
+        } both{read_tr16(&bs[swn<BN>(kr, nc)]), read_tr16(&bs[swn<BN>(kr + 4, nc)])};
+        bfr[j] = __builtin_bit_cast(bf16x8, both);
+      }
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    }
+  };
+  constexpr int L = AJ + BJ;  // loads per lane per stage
+  // The barrier carries no memory fence (a workgroup fence would wait for every load in
+  // flight, the next stage's too), so scheduling barriers keep every LDS read and MFMA of a
+  // stage on its side of the wait and the barrier.
+  auto step = [&]() {
+    __builtin_amdgcn_sched_barrier(0);
+    wait_vmcnt<L>();
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_sched_barrier(0);
+  };
+
+  const int nk = (K + BK - 1) / BK;
+  __syncthreads();  // LDS may still be read by the previous tile of a persistent loop
+  issue(sA0, sB0, 0);
+  issue(sA1, sB1, BK);
+  step();
+  // whole triples of stages, buffers 0, 1, 2 in turn; then the last one or two stages
+  int kt = 0;
+  for (; kt + 2 < nk; kt += 3) {
+    issue(sA2, sB2, (kt + 2) * BK);
+    compute(sA0, sB0);
+    step();
+    issue(sA0, sB0, (kt + 3) * BK);
+    compute(sA1, sB1);
+    step();
+    issue(sA1, sB1, (kt + 4) * BK);
+    compute(sA2, sB2);
+    step();
+  }
+  if (kt < nk) {  // stage kt is in buffer 0, stage kt + 1 (if any) in buffer 1
+    compute(sA0, sB0);
+    if (kt + 1 < nk) {
+      // stage kt + 1's loads are the newest in flight: wait for all of them before the barrier
+      __builtin_amdgcn_sched_barrier(0);
+      wait_vmcnt<0>();
+      __builtin_amdgcn_s_barrier();
+      __builtin_amdgcn_sched_barrier(0);
+      compute(sA1, sB1);
+    }
+  }
+  wait_vmcnt<0>();  // stages issued past K have landed before the next tile reuses LDS
+
+  const __amdgpu_buffer_rsrc_t rC = make_rsrc(op.C, ((int64_t)(M - 1) * op.ldc + N) * 4);
+  const float alpha = op.alpha;
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int gn = n0 + wn * WN + j * 32 + col;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int gm = m0 + wm * WM + i * 32 + (v >> 2) * 8 + half * 4 + (v & 3);
+        const uint32_t off = (gm < M && gn < N) ? (uint32_t)(((int64_t)gm * op.ldc + gn) * 4) : kOOB;
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, alpha * acc[i][j][v]), rC, off, 0u, 0);
+      }
+    }
+}
+
+// BF16 tiles that take the LDS-direct stage loads
+template <int BM, int BN, typename InT, int S, int T>
+struct uses_dma {
+  static constexpr bool value = sizeof(InT) == 2 && S == 1 && T == 1 && BM == 256;
+};
+
 // Compute one BM x BN tile at (m0, n0).  smem must hold smem_layout<BM,BN,S>::bytes.
 template <int BM, int BN, typename InT, int S, int T, bool FAST>
 __device__ __forceinline__ void tile(const op_view &op, int m0, int n0, uint16_t *smem) {
-  if constexpr (FAST && ((sizeof(InT) == 2 && S == 1 && T == 1) || (sizeof(InT) == 4 && S >= 2 && BM * BN <= 256 * 128))) {
+  if constexpr (FAST && uses_dma<BM, BN, InT, S, T>::value) {
+    tile_dma<BM, BN>(op, m0, n0);
+    return;
+  } else if constexpr (FAST && ((sizeof(InT) == 2 && S == 1 && T == 1) ||
+                                (sizeof(InT) == 4 && S >= 2 && BM * BN <= 256 * 128))) {
     tile_fast<BM, BN, InT, S, T>(op, m0, n0, smem);
     return;
   }

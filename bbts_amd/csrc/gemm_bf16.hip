@@ -156,8 +156,20 @@ int smem_bytes(int cfg, int variant) {
   return 2 * S * (kCfgs[cfg].bm + kCfgs[cfg].bn) * stage_k_of(cfg, variant) * 2;
 }
 
-// a config fits when its two LDS stages fit the 160 KiB of a CU
-bool valid_cfg(int cfg, int variant) { return smem_bytes(cfg, variant) <= 160 * 1024; }
+// BF16 tiles of 256 rows load their stages straight into three static LDS stages
+// (tile_dma); everything else takes two stages of dynamic LDS
+bool dma_cfg(int cfg, int variant) { return variant == 0 && kCfgs[cfg].bm == 256; }
+
+// LDS a workgroup of the config holds (static or dynamic)
+int lds_bytes(int cfg, int variant) {
+  return dma_cfg(cfg, variant) ? 3 * (kCfgs[cfg].bm + kCfgs[cfg].bn) * 32 * 2 : smem_bytes(cfg, variant);
+}
+
+// dynamic LDS to launch the kernel with
+int dyn_smem_bytes(int cfg, int variant, bool fast) { return fast && dma_cfg(cfg, variant) ? 0 : smem_bytes(cfg, variant); }
+
+// a config fits when its LDS stages fit the 160 KiB of a CU
+bool valid_cfg(int cfg, int variant) { return lds_bytes(cfg, variant) <= 160 * 1024; }
 
 // the element-wise (non-FAST) path is built for the 128x128 and 64x64 tiles only
 int generic_cfg(int cfg) { return (kCfgs[cfg].bm >= 128 && kCfgs[cfg].bn >= 128) ? 1 : 3; }
@@ -201,7 +213,7 @@ int ensure_smem(const void *fn, int bytes) {
 
 // Workgroups of one config that fit on a CU (LDS-limited; 160 KiB per CU).
 int occupancy(int cfg, int variant) {
-  int b = smem_bytes(cfg, variant) + 64;
+  int b = lds_bytes(cfg, variant) + 64;
   int by_lds = (160 * 1024) / b;
   // waves per SIMD the register budget allows (4 waves per workgroup, one per SIMD)
   static const int kByRegs[] = {variant == 0 ? 2 : 1, 2, 3, 4, 3, 1};
@@ -303,7 +315,7 @@ int launch_gemm_split_bf16(bbts_hip_ctx *ctx, const gemm_args &g, int slices_a, 
   if (!valid_cfg(cfg, variant)) cfg = 1;
   int tiles_m = (g.M + kCfgs[cfg].bm - 1) / kCfgs[cfg].bm, tiles_n = (g.N + kCfgs[cfg].bn - 1) / kCfgs[cfg].bn;
   single_fn fn = get_single(cfg, variant, fast);
-  int bytes = smem_bytes(cfg, variant);
+  int bytes = dyn_smem_bytes(cfg, variant, fast);
   if ((rc = ensure_smem((const void *)fn, bytes))) return rc;
   hipLaunchKernelGGL(fn, dim3((unsigned)(tiles_m * tiles_n), 1, (unsigned)g.batch), dim3(256), (unsigned)bytes, s, v,
                      sa, sb, g.sc, tiles_m, tiles_n, g.meta_dst, g.w0, g.w1);
@@ -476,7 +488,7 @@ int bbts_hip_gemm_desc_launch(bbts_hip_ctx *ctx, const bbts_gemm_desc_t *d, int 
   if (!fast) cfg = generic_cfg(cfg);
   int tiles_m = (d->M + kCfgs[cfg].bm - 1) / kCfgs[cfg].bm, tiles_n = (d->N + kCfgs[cfg].bn - 1) / kCfgs[cfg].bn;
   single_fn fn = get_single(cfg, variant, fast);
-  int bytes = smem_bytes(cfg, variant);
+  int bytes = dyn_smem_bytes(cfg, variant, fast);
   if ((rc = ensure_smem((const void *)fn, bytes))) return rc;
   hipLaunchKernelGGL(fn, dim3((unsigned)(tiles_m * tiles_n)), dim3(256), (unsigned)bytes, (hipStream_t)stream, v,
                      (int64_t)0, (int64_t)0, (int64_t)0, tiles_m, tiles_n, (uint32_t *)nullptr, 0u, 0u);
@@ -742,7 +754,7 @@ int bbts_hip_batch_launch_group(bbts_batch *b, int g, bbts_stream_t stream) {
   const auto &grp = b->groups[(size_t)g];
   char *base = static_cast<char *>(b->dev);
   grouped_fn fn = get_grouped(grp.cfg, grp.variant, grp.fast);
-  int bytes = smem_bytes(grp.cfg, grp.variant);
+  int bytes = dyn_smem_bytes(grp.cfg, grp.variant, grp.fast);
   if ((rc = ensure_smem((const void *)fn, bytes))) return rc;
   hipLaunchKernelGGL(fn, dim3((unsigned)grp.grid), dim3(256), (unsigned)bytes, (hipStream_t)stream,
                      reinterpret_cast<const gop *>(base + grp.ops_off),
@@ -784,7 +796,7 @@ int bbts_hip_batch_capture(bbts_batch *b, bbts_stream_t stream) {
   // kernel attributes must be set outside capture
   for (auto &grp : b->groups)
     if ((rc = ensure_smem((const void *)get_grouped(grp.cfg, grp.variant, grp.fast),
-                          smem_bytes(grp.cfg, grp.variant))))
+                          dyn_smem_bytes(grp.cfg, grp.variant, grp.fast))))
       return rc;
   hipStream_t s = (hipStream_t)stream;
   hipStream_t cap = s;
