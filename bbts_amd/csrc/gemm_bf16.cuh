@@ -29,6 +29,8 @@
 #pragma once
 #include "common.h"
 
+#include <type_traits>
+
 namespace bbts_hip {
 namespace bf16gemm {
 
@@ -370,29 +372,62 @@ __device__ __forceinline__ void wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt((n & 15) | (7 << 4) | (15 << 8) | ((n >> 4) << 14));
 }
 
+// Compile-time loop: f(integral_constant<int, I>) for I in [I0, N)
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    static_for<I + 1, N>(f);
+  }
+}
+
+// LDS stages of the LDS-direct tiles: three (256 x 256: 96 KiB, one workgroup per CU;
+// 256 x 128: 72 KiB, two per CU).  A fourth stage for 256 x 256 (128 KiB) measured 2-4 %
+// slower on single launches and on the 512-op stream.
+template <int BM, int BN>
+struct dma_stages {
+  static constexpr int value = 3;
+};
+
 // BF16 tiles of 256 rows: the operand stages go from HBM straight into LDS (buffer loads with
 // the LDS-destination bit; gfx950 moves 16 bytes per lane, lane l of an instruction writing
 // the 16 bytes at the instruction's LDS base + 16 l).  No register staging and no LDS write
-// instructions: the VGPRs the two prefetch register sets held are free, and the issue slots
-// the LDS writes took go to the MFMA pipe.  Three LDS stages keep two stages of loads in
-// flight; each lane loads the global chunk whose swizzled position is its LDS slot (the XOR
-// swizzles of sw / swn are involutions), so the compute reads are those of tile_fast.
-// Synchronisation per stage: issue stage k+2, compute stage k, wait until at most stage k+2's
-// loads are outstanding (this wave's share of stage k+1 has landed), then a barrier without a
-// fence (every wave's share of stage k+1 has landed and stage k's buffer is free).
+// instructions: the VGPRs the prefetch register sets held are free, and the issue slots the
+// LDS writes took go to the MFMA pipe.  NST LDS stages keep NST - 1 stages of loads in flight;
+// each lane loads the global chunk whose swizzled position is its LDS slot (the XOR swizzles
+// of sw / swn are involutions), so the compute reads are those of tile_fast.  Each stage
+// buffer is an LDS object of its own, so the compiler can tell the stages' loads apart.
+// Synchronisation per stage: issue stage k + NST - 1, compute stage k, wait until at most the
+// newest NST - 2 stages' loads are outstanding (this wave's share of stage k + 1 has landed),
+// then a barrier without a fence (every wave's share of stage k + 1 has landed and stage k's
+// buffer is free).
 template <int BM, int BN>
 __device__ __forceinline__ void tile_dma(const op_view &op, int m0, int n0) {
   constexpr int BK = stage_k<BM, BN, 1>::value;
-  static_assert(BK == 32, "three 32-deep stages");
+  constexpr int NST = dma_stages<BM, BN>::value;
+  static_assert(BK == 32 && NST >= 3 && NST <= 4, "three or four 32-deep stages");
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 32, FN = WN / 32;
   constexpr int CPR = BK / 8;          // 16-byte chunks per A row
   constexpr int CPB = BN / 8;          // 16-byte chunks per B k-row
   constexpr int AJ = BM * CPR / 256;   // A loads per lane per stage
   constexpr int BJ = BK * CPB / 256;   // B loads per lane per stage
+  constexpr int L = AJ + BJ;           // loads per lane per stage
   static_assert(AJ * 256 == BM * CPR && BJ * 256 == BK * CPB, "whole instructions per stage");
-  __shared__ __align__(16) uint16_t sA0[BM * BK], sA1[BM * BK], sA2[BM * BK];
-  __shared__ __align__(16) uint16_t sB0[BK * BN], sB1[BK * BN], sB2[BK * BN];
+  __shared__ __align__(16) uint16_t sA0[BM * BK], sA1[BM * BK], sA2[BM * BK], sA3[NST > 3 ? BM * BK : 8];
+  __shared__ __align__(16) uint16_t sB0[BK * BN], sB1[BK * BN], sB2[BK * BN], sB3[NST > 3 ? BK * BN : 8];
+  auto bufA = [&](auto I) -> uint16_t * {
+    if constexpr (decltype(I)::value == 0) return sA0;
+    else if constexpr (decltype(I)::value == 1) return sA1;
+    else if constexpr (decltype(I)::value == 2) return sA2;
+    else return sA3;
+  };
+  auto bufB = [&](auto I) -> uint16_t * {
+    if constexpr (decltype(I)::value == 0) return sB0;
+    else if constexpr (decltype(I)::value == 1) return sB1;
+    else if constexpr (decltype(I)::value == 2) return sB2;
+    else return sB3;
+  };
 
   const int M = op.M, N = op.N, K = op.K;
   const int tid = threadIdx.x, lane = tid & 63;
@@ -420,10 +455,11 @@ __device__ __forceinline__ void tile_dma(const op_view &op, int m0, int n0) {
     vb[j] = gn < N ? (uint32_t)(((int64_t)krow * op.ldb + gn) * 2) : kOOB;
     kb[j] = krow;
   }
-  // Stage at k0 into (sa, sb).  Lanes past M, N or K get the out-of-range offset (the
+  // Stage at k0 into buffer I.  Lanes past M, N or K get the out-of-range offset (the
   // descriptor's range check writes zeros and reads nothing); a stage at or past K reads
   // nothing and has a zero scalar offset, so no offset can wrap back into range.
-  auto issue = [&](uint16_t *sa, uint16_t *sb, int k0) {
+  auto issue = [&](auto I, int k0) {
+    uint16_t *sa = bufA(I), *sb = bufB(I);
     const int lim = K - k0;
     const uint32_t soa = k0 < K ? (uint32_t)k0 * 2u : 0u;
     const uint32_t sob = k0 < K ? (uint32_t)((int64_t)k0 * op.ldb * 2) : 0u;
@@ -446,7 +482,8 @@ __device__ __forceinline__ void tile_dma(const op_view &op, int m0, int n0) {
       for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
 
   const int half = lane >> 5, col = lane & 31;
-  auto compute = [&](const uint16_t *as, const uint16_t *bs) {
+  auto compute = [&](auto I) {
+    const uint16_t *as = bufA(I), *bs = bufB(I);
     __builtin_amdgcn_iglp_opt(0);
 #pragma unroll
     for (int kk = 0; kk < BK / 16; ++kk) {
@@ -472,46 +509,40 @@ __device__ __forceinline__ void tile_dma(const op_view &op, int m0, int n0) {
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
     }
   };
-  constexpr int L = AJ + BJ;  // loads per lane per stage
   // The barrier carries no memory fence (a workgroup fence would wait for every load in
-  // flight, the next stage's too), so scheduling barriers keep every LDS read and MFMA of a
+  // flight, the later stages' too), so scheduling barriers keep every LDS read and MFMA of a
   // stage on its side of the wait and the barrier.
-  auto step = [&]() {
+  auto step = [&](auto W) {
     __builtin_amdgcn_sched_barrier(0);
-    wait_vmcnt<L>();
+    wait_vmcnt<decltype(W)::value>();
     __builtin_amdgcn_s_barrier();
     __builtin_amdgcn_sched_barrier(0);
   };
+  using in_flight = std::integral_constant<int, (NST - 2) * L>;  // newest NST - 2 stages
+  using none = std::integral_constant<int, 0>;
 
   const int nk = (K + BK - 1) / BK;
   __syncthreads();  // LDS may still be read by the previous tile of a persistent loop
-  issue(sA0, sB0, 0);
-  issue(sA1, sB1, BK);
-  step();
-  // whole triples of stages, buffers 0, 1, 2 in turn; then the last one or two stages
+  static_for<0, NST - 1>([&](auto I) { issue(I, decltype(I)::value * BK); });
+  step(in_flight{});
+  // whole groups of NST stages, buffers 0 .. NST-1 in turn; then the last stages
   int kt = 0;
-  for (; kt + 2 < nk; kt += 3) {
-    issue(sA2, sB2, (kt + 2) * BK);
-    compute(sA0, sB0);
-    step();
-    issue(sA0, sB0, (kt + 3) * BK);
-    compute(sA1, sB1);
-    step();
-    issue(sA1, sB1, (kt + 4) * BK);
-    compute(sA2, sB2);
-    step();
+  for (; kt + NST - 1 < nk; kt += NST) {
+    static_for<0, NST>([&](auto R) {
+      constexpr int r = decltype(R)::value;
+      issue(std::integral_constant<int, (r + NST - 1) % NST>{}, (kt + r + NST - 1) * BK);
+      compute(R);
+      step(in_flight{});
+    });
   }
-  if (kt < nk) {  // stage kt is in buffer 0, stage kt + 1 (if any) in buffer 1
-    compute(sA0, sB0);
-    if (kt + 1 < nk) {
-      // stage kt + 1's loads are the newest in flight: wait for all of them before the barrier
-      __builtin_amdgcn_sched_barrier(0);
-      wait_vmcnt<0>();
-      __builtin_amdgcn_s_barrier();
-      __builtin_amdgcn_sched_barrier(0);
-      compute(sA1, sB1);
+  // stage kt + r sits in buffer r; after the first, each waits for every load in flight
+  static_for<0, NST - 1>([&](auto R) {
+    constexpr int r = decltype(R)::value;
+    if (kt + r < nk) {
+      if constexpr (r > 0) step(none{});
+      compute(R);
     }
-  }
+  });
   wait_vmcnt<0>();  // stages issued past K have landed before the next tile reuses LDS
 
   const __amdgpu_buffer_rsrc_t rC = make_rsrc(op.C, ((int64_t)(M - 1) * op.ldc + N) * 4);

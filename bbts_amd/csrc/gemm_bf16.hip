@@ -156,13 +156,14 @@ int smem_bytes(int cfg, int variant) {
   return 2 * S * (kCfgs[cfg].bm + kCfgs[cfg].bn) * stage_k_of(cfg, variant) * 2;
 }
 
-// BF16 tiles of 256 rows load their stages straight into three static LDS stages
+// BF16 tiles of 256 rows load their stages straight into three or four static LDS stages
 // (tile_dma); everything else takes two stages of dynamic LDS
 bool dma_cfg(int cfg, int variant) { return variant == 0 && kCfgs[cfg].bm == 256; }
 
 // LDS a workgroup of the config holds (static or dynamic)
 int lds_bytes(int cfg, int variant) {
-  return dma_cfg(cfg, variant) ? 3 * (kCfgs[cfg].bm + kCfgs[cfg].bn) * 32 * 2 : smem_bytes(cfg, variant);
+  const int nst = 3;  // dma_stages
+  return dma_cfg(cfg, variant) ? nst * (kCfgs[cfg].bm + kCfgs[cfg].bn) * 32 * 2 : smem_bytes(cfg, variant);
 }
 
 // dynamic LDS to launch the kernel with
