@@ -401,19 +401,21 @@ struct dma_stages {
 // newest NST - 2 stages' loads are outstanding (this wave's share of stage k + 1 has landed),
 // then a barrier without a fence (every wave's share of stage k + 1 has landed and stage k's
 // buffer is free).
-template <int BM, int BN>
+template <int BM, int BN, int NT>
 __device__ __forceinline__ void tile_dma(const op_view &op, int m0, int n0) {
   constexpr int BK = stage_k<BM, BN, 1>::value;
   constexpr int NST = dma_stages<BM, BN>::value;
   static_assert(BK == 32 && NST >= 3 && NST <= 4, "three or four 32-deep stages");
-  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int NW = NT / 64;          // waves: 2 x 2 (256 lanes) or 2 x 4 (512 lanes)
+  constexpr int WGN = NW / 2;
+  constexpr int WM = BM / 2, WN = BN / WGN;
   constexpr int FM = WM / 32, FN = WN / 32;
   constexpr int CPR = BK / 8;          // 16-byte chunks per A row
   constexpr int CPB = BN / 8;          // 16-byte chunks per B k-row
-  constexpr int AJ = BM * CPR / 256;   // A loads per lane per stage
-  constexpr int BJ = BK * CPB / 256;   // B loads per lane per stage
+  constexpr int AJ = BM * CPR / NT;    // A loads per lane per stage
+  constexpr int BJ = BK * CPB / NT;    // B loads per lane per stage
   constexpr int L = AJ + BJ;           // loads per lane per stage
-  static_assert(AJ * 256 == BM * CPR && BJ * 256 == BK * CPB, "whole instructions per stage");
+  static_assert(AJ * NT == BM * CPR && BJ * NT == BK * CPB && FN >= 1, "whole instructions per stage");
   __shared__ __align__(16) uint16_t sA0[BM * BK], sA1[BM * BK], sA2[BM * BK], sA3[NST > 3 ? BM * BK : 8];
   __shared__ __align__(16) uint16_t sB0[BK * BN], sB1[BK * BN], sB2[BK * BN], sB3[NST > 3 ? BK * BN : 8];
   auto bufA = [&](auto I) -> uint16_t * {
@@ -432,16 +434,16 @@ __device__ __forceinline__ void tile_dma(const op_view &op, int m0, int n0) {
   const int M = op.M, N = op.N, K = op.K;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave / WGN, wn = wave % WGN;
   const __amdgpu_buffer_rsrc_t rA = make_rsrc(op.A, ((int64_t)(M - 1) * op.lda + K) * 2);
   const __amdgpu_buffer_rsrc_t rB = make_rsrc(op.B, ((int64_t)(K - 1) * op.ldb + N) * 2);
 
-  // lane's chunks: instruction j of wave w fills LDS chunks (4 j + w) * 64 + lane
+  // lane's chunks: instruction j of wave w fills LDS chunks (NW j + w) * 64 + lane
   uint32_t va[AJ], vb[BJ];
   int ka[AJ], kb[BJ];
 #pragma unroll
   for (int j = 0; j < AJ; ++j) {
-    const int p = (j * 4 + wave) * 64 + lane, row = p / CPR, slot = p % CPR;
+    const int p = (j * NW + wave) * 64 + lane, row = p / CPR, slot = p % CPR;
     const int kc = slot ^ ((row >> 2) & 3);  // sw<32>: the logical k chunk stored in this slot
     const int gm = m0 + row;
     va[j] = gm < M ? (uint32_t)(((int64_t)gm * op.lda + kc * 8) * 2) : kOOB;
@@ -449,7 +451,7 @@ __device__ __forceinline__ void tile_dma(const op_view &op, int m0, int n0) {
   }
 #pragma unroll
   for (int j = 0; j < BJ; ++j) {
-    const int p = (j * 4 + wave) * 64 + lane, krow = p / CPB, slot = p % CPB;
+    const int p = (j * NW + wave) * 64 + lane, krow = p / CPB, slot = p % CPB;
     const int nc = slot ^ (((krow & 3) << 2) & (CPB - 1));  // swn<BN>
     const int gn = n0 + nc * 8;
     vb[j] = gn < N ? (uint32_t)(((int64_t)krow * op.ldb + gn) * 2) : kOOB;
@@ -465,11 +467,11 @@ __device__ __forceinline__ void tile_dma(const op_view &op, int m0, int n0) {
     const uint32_t sob = k0 < K ? (uint32_t)((int64_t)k0 * op.ldb * 2) : 0u;
 #pragma unroll
     for (int j = 0; j < AJ; ++j)
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(rA, (lds_void_t *)(sa + (j * 4 + wave) * 64 * 8), 16,
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rA, (lds_void_t *)(sa + (j * NW + wave) * 64 * 8), 16,
                                                ka[j] < lim ? va[j] : kOOB, soa, 0, 0);
 #pragma unroll
     for (int j = 0; j < BJ; ++j)
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(rB, (lds_void_t *)(sb + (j * 4 + wave) * 64 * 8), 16,
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rB, (lds_void_t *)(sb + (j * NW + wave) * 64 * 8), 16,
                                                kb[j] < lim ? vb[j] : kOOB, sob, 0, 0);
   };
 
@@ -567,11 +569,19 @@ struct uses_dma {
   static constexpr bool value = sizeof(InT) == 2 && S == 1 && T == 1 && BM == 256;
 };
 
+// Lanes per workgroup: 512 (8 waves, two per SIMD, 128 x 64 each) for the LDS-direct
+// 256 x 256 tile, so that one wave's MFMAs fill the other's LDS-read and barrier waits
+// within the CU's one workgroup; 256 otherwise.
+template <int BM, int BN, typename InT, int S, int T, bool FAST>
+struct tile_threads {
+  static constexpr int value = (FAST && uses_dma<BM, BN, InT, S, T>::value && BN == 256) ? 512 : 256;
+};
+
 // Compute one BM x BN tile at (m0, n0).  smem must hold smem_layout<BM,BN,S>::bytes.
 template <int BM, int BN, typename InT, int S, int T, bool FAST>
 __device__ __forceinline__ void tile(const op_view &op, int m0, int n0, uint16_t *smem) {
   if constexpr (FAST && uses_dma<BM, BN, InT, S, T>::value) {
-    tile_dma<BM, BN>(op, m0, n0);
+    tile_dma<BM, BN, tile_threads<BM, BN, InT, S, T, FAST>::value>(op, m0, n0);
     return;
   } else if constexpr (FAST && ((sizeof(InT) == 2 && S == 1 && T == 1) ||
                                 (sizeof(InT) == 4 && S >= 2 && BM * BN <= 256 * 128))) {

@@ -70,8 +70,14 @@ struct waves_target {
           : 1;
 };
 
+// waves per SIMD of the kernel a config builds (the 512-lane 256 x 256 tile holds two)
 template <int BM, int BN, typename InT, int S, int T, bool FAST>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_target<BM, BN, InT, S>::value))) void gemm_single_kernel(op_view op, int64_t sa, int64_t sb, int64_t sc, int tiles_m,
+struct kernel_waves {
+  static constexpr int value = tile_threads<BM, BN, InT, S, T, FAST>::value == 512 ? 2 : waves_target<BM, BN, InT, S>::value;
+};
+
+template <int BM, int BN, typename InT, int S, int T, bool FAST>
+__global__ __launch_bounds__((tile_threads<BM, BN, InT, S, T, FAST>::value)) __attribute__((amdgpu_waves_per_eu(kernel_waves<BM, BN, InT, S, T, FAST>::value))) void gemm_single_kernel(op_view op, int64_t sa, int64_t sb, int64_t sc, int tiles_m,
                                                           int tiles_n, uint32_t *meta_dst, uint32_t w0, uint32_t w1) {
   extern __shared__ __align__(16) uint16_t smem[];
   if (meta_dst != nullptr && blockIdx.x == 0 && blockIdx.z == 0 && threadIdx.x == 0) {
@@ -90,7 +96,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_targe
 // Queue x of nq holds items [qstart[x], qstart[x+1]); workgroup w serves queue w % nq
 // (the grid is a multiple of nq), with its own ticket counter at counters[64 x].
 template <int BM, int BN, typename InT, int S, int T, bool FAST>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_target<BM, BN, InT, S>::value))) void gemm_grouped_kernel(const gop *__restrict__ ops, const int2 *__restrict__ items,
+__global__ __launch_bounds__((tile_threads<BM, BN, InT, S, T, FAST>::value)) __attribute__((amdgpu_waves_per_eu(kernel_waves<BM, BN, InT, S, T, FAST>::value))) void gemm_grouped_kernel(const gop *__restrict__ ops, const int2 *__restrict__ items,
                                                            const int *__restrict__ qstart, int nq,
                                                            unsigned *counters) {
   extern __shared__ __align__(16) uint16_t smem[];
@@ -165,6 +171,9 @@ int lds_bytes(int cfg, int variant) {
   const int nst = 3;  // dma_stages
   return dma_cfg(cfg, variant) ? nst * (kCfgs[cfg].bm + kCfgs[cfg].bn) * 32 * 2 : smem_bytes(cfg, variant);
 }
+
+// lanes per workgroup (tile_threads): 512 for the LDS-direct 256 x 256 BF16 tile
+int threads_for(int cfg, int variant, bool fast) { return fast && dma_cfg(cfg, variant) && kCfgs[cfg].bn == 256 ? 512 : 256; }
 
 // dynamic LDS to launch the kernel with
 int dyn_smem_bytes(int cfg, int variant, bool fast) { return fast && dma_cfg(cfg, variant) ? 0 : smem_bytes(cfg, variant); }
@@ -318,7 +327,8 @@ int launch_gemm_split_bf16(bbts_hip_ctx *ctx, const gemm_args &g, int slices_a, 
   single_fn fn = get_single(cfg, variant, fast);
   int bytes = dyn_smem_bytes(cfg, variant, fast);
   if ((rc = ensure_smem((const void *)fn, bytes))) return rc;
-  hipLaunchKernelGGL(fn, dim3((unsigned)(tiles_m * tiles_n), 1, (unsigned)g.batch), dim3(256), (unsigned)bytes, s, v,
+  hipLaunchKernelGGL(fn, dim3((unsigned)(tiles_m * tiles_n), 1, (unsigned)g.batch), dim3(threads_for(cfg, variant, fast)),
+                     (unsigned)bytes, s, v,
                      sa, sb, g.sc, tiles_m, tiles_n, g.meta_dst, g.w0, g.w1);
   BBTS_LAUNCH_CHECK();
   if (tmpA || tmpB) {
@@ -491,7 +501,8 @@ int bbts_hip_gemm_desc_launch(bbts_hip_ctx *ctx, const bbts_gemm_desc_t *d, int 
   single_fn fn = get_single(cfg, variant, fast);
   int bytes = dyn_smem_bytes(cfg, variant, fast);
   if ((rc = ensure_smem((const void *)fn, bytes))) return rc;
-  hipLaunchKernelGGL(fn, dim3((unsigned)(tiles_m * tiles_n)), dim3(256), (unsigned)bytes, (hipStream_t)stream, v,
+  hipLaunchKernelGGL(fn, dim3((unsigned)(tiles_m * tiles_n)), dim3(threads_for(cfg, variant, fast)), (unsigned)bytes,
+                     (hipStream_t)stream, v,
                      (int64_t)0, (int64_t)0, (int64_t)0, tiles_m, tiles_n, (uint32_t *)nullptr, 0u, 0u);
   BBTS_LAUNCH_CHECK();
   return BBTS_OK;
@@ -757,7 +768,8 @@ int bbts_hip_batch_launch_group(bbts_batch *b, int g, bbts_stream_t stream) {
   grouped_fn fn = get_grouped(grp.cfg, grp.variant, grp.fast);
   int bytes = dyn_smem_bytes(grp.cfg, grp.variant, grp.fast);
   if ((rc = ensure_smem((const void *)fn, bytes))) return rc;
-  hipLaunchKernelGGL(fn, dim3((unsigned)grp.grid), dim3(256), (unsigned)bytes, (hipStream_t)stream,
+  hipLaunchKernelGGL(fn, dim3((unsigned)grp.grid), dim3(threads_for(grp.cfg, grp.variant, grp.fast)), (unsigned)bytes,
+                     (hipStream_t)stream,
                      reinterpret_cast<const gop *>(base + grp.ops_off),
                      reinterpret_cast<const int2 *>(base + grp.items_off),
                      reinterpret_cast<const int *>(base + grp.qstart_off), grp.nq,
