@@ -488,7 +488,6 @@ __device__ __forceinline__ void tile_dma(const op_view &op, int m0, int n0) {
   // 1-4 % faster than strategy 0
   auto compute = [&](auto I) {
     const uint16_t *as = bufA(I), *bs = bufB(I);
-    
 #pragma unroll
     for (int kk = 0; kk < BK / 16; ++kk) {
       const int k = kk * 16 + half * 8;
