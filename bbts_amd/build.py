@@ -40,8 +40,11 @@ def _run(cmd):
 
 
 # per-source flags after HIP_FLAGS: the cutensor elementwise kernels round alpha * lhs and
-# beta * rhs on their own before the binary op, as the reference's separate MKL calls do
-PER_SOURCE_FLAGS = {"cutensor_ew.hip": ["-ffp-contract=off"]}
+# beta * rhs on their own before the binary op, as the reference's separate MKL calls do; the
+# grouped GEMM's ticket atomics stay single-lane atomics (the wave-aggregating rewrite waits
+# for the result at once, which drains the previous tile's stores at every tile boundary)
+PER_SOURCE_FLAGS = {"cutensor_ew.hip": ["-ffp-contract=off"],
+                    "gemm_bf16.hip": ["-mllvm", "-amdgpu-atomic-optimizer-strategy=None", "-mllvm", "-structurizecfg-skip-uniform-regions"]}
 
 
 def _compile(src, extra=()):

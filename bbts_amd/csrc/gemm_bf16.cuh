@@ -68,6 +68,7 @@ __device__ __forceinline__ int swn(int k, int n) {
 }
 
 typedef short v4s __attribute__((ext_vector_type(4)));
+typedef unsigned int v4u32 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) v4s lds_v4s;
 
 // ds_read_b64_tr_b16: within each 16-lane group, source lane j addresses 4 consecutive
@@ -154,6 +155,10 @@ inline bool fast_ok(const op_view &v, int epv) {
 // before an LDS write leaves the newest stage's loads in flight.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kOOB = 0x80000000u;
+// Cache policy of the C stores: non-temporal.  Each C element is written once and not read
+// again by the launch, so it should not displace the A and B panels that neighbouring tiles
+// re-read from L2 (+3 % on a stream of 512 uneven GEMMs).
+constexpr int kCStorePolicy = 2;
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, int64_t bytes) {
   // raw buffer (stride 0); dword3 0x00020000 is the gfx9 data-format word for untyped access
@@ -357,7 +362,8 @@ __device__ __forceinline__ void tile_fast(const op_view &op, int m0, int n0, uin
       for (int v = 0; v < 16; ++v) {
         const int gm = m0 + wm * WM + i * 32 + (v >> 2) * 8 + half * 4 + (v & 3);
         const uint32_t off = (gm < M && gn < N) ? (uint32_t)(((int64_t)gm * op.ldc + gn) * 4) : kOOB;
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, alpha * acc[i][j][v]), rC, off, 0u, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, alpha * acc[i][j][v]), rC, off, 0u,
+                                              kCStorePolicy);
       }
     }
 }
@@ -372,99 +378,102 @@ __device__ __forceinline__ void wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt((n & 15) | (7 << 4) | (15 << 8) | ((n >> 4) << 14));
 }
 
-// Compile-time loop: f(integral_constant<int, I>) for I in [I0, N)
-template <int I, int N, typename F>
-__device__ __forceinline__ void static_for(F &&f) {
-  if constexpr (I < N) {
-    f(std::integral_constant<int, I>{});
-    static_for<I + 1, N>(f);
-  }
+// Workgroup barrier for LDS traffic only: this wave's LDS accesses complete (lgkmcnt 0), then
+// s_barrier, with no wait on vector memory, so global stores and loads in flight stay in
+// flight across it.  The empty asm statements keep the compiler from moving memory accesses
+// across it; the scheduling barriers do the same for the machine scheduler.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_sched_barrier(0);
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(15 | (7 << 4) | (0 << 8) | (3 << 14));  // lgkmcnt(0) only
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_sched_barrier(0);
 }
-
-// LDS stages of the LDS-direct tiles: three (256 x 256: 96 KiB, one workgroup per CU;
-// 256 x 128: 72 KiB, two per CU).  A fourth stage for 256 x 256 (128 KiB) measured 2-4 %
-// slower on single launches and on the 512-op stream.
-template <int BM, int BN>
-struct dma_stages {
-  static constexpr int value = 3;
-};
 
 // BF16 tiles of 256 rows: the operand stages go from HBM straight into LDS (buffer loads with
 // the LDS-destination bit; gfx950 moves 16 bytes per lane, lane l of an instruction writing
 // the 16 bytes at the instruction's LDS base + 16 l).  No register staging and no LDS write
-// instructions: the VGPRs the prefetch register sets held are free, and the issue slots the
-// LDS writes took go to the MFMA pipe.  NST LDS stages keep NST - 1 stages of loads in flight;
-// each lane loads the global chunk whose swizzled position is its LDS slot (the XOR swizzles
-// of sw / swn are involutions), so the compute reads are those of tile_fast.  Each stage
-// buffer is an LDS object of its own, so the compiler can tell the stages' loads apart.
-// Synchronisation per stage: issue stage k + NST - 1, compute stage k, wait until at most the
-// newest NST - 2 stages' loads are outstanding (this wave's share of stage k + 1 has landed),
-// then a barrier without a fence (every wave's share of stage k + 1 has landed and stage k's
-// buffer is free).
+// instructions: the VGPRs of the prefetch register sets are free, and the issue slots the LDS
+// writes took go to the MFMA pipe.  Three 32-deep LDS stages keep two stages of loads in
+// flight; each lane loads the global chunk whose swizzled position is its LDS slot (the XOR
+// swizzles of sw / swn are involutions), so the MFMA-side reads are those of tile_fast.  Each
+// stage buffer is an LDS object of its own, so the compiler can tell the stages' loads apart.
+// A fourth stage, or two 64-deep stages, measured 2-4 % slower.
+//
+// Per stage: issue stage k + 2, compute stage k, wait until at most stage k + 2's loads are
+// outstanding (this wave's share of stage k + 1 has landed), then a barrier without a fence
+// (every wave's share of stage k + 1 has landed and stage k's buffer is free).  Waves: 2 x 2
+// (256 lanes) or 2 x 4 (512 lanes, two waves per SIMD, for 256 x 256).
 template <int BM, int BN, int NT>
-__device__ __forceinline__ void tile_dma(const op_view &op, int m0, int n0) {
-  constexpr int BK = stage_k<BM, BN, 1>::value;
-  constexpr int NST = dma_stages<BM, BN>::value;
-  static_assert(BK == 32 && NST >= 3 && NST <= 4, "three or four 32-deep stages");
-  constexpr int NW = NT / 64;          // waves: 2 x 2 (256 lanes) or 2 x 4 (512 lanes)
-  constexpr int WGN = NW / 2;
-  constexpr int WM = BM / 2, WN = BN / WGN;
-  constexpr int FM = WM / 32, FN = WN / 32;
-  constexpr int CPR = BK / 8;          // 16-byte chunks per A row
-  constexpr int CPB = BN / 8;          // 16-byte chunks per B k-row
-  constexpr int AJ = BM * CPR / NT;    // A loads per lane per stage
-  constexpr int BJ = BK * CPB / NT;    // B loads per lane per stage
-  constexpr int L = AJ + BJ;           // loads per lane per stage
+struct dma_tile {
+  static constexpr int BK = stage_k<BM, BN, 1>::value;
+  static_assert(BK == 32, "32-deep stages");
+  static constexpr int NW = NT / 64, WGN = NW / 2;
+  static constexpr int WM = BM / 2, WN = BN / WGN;
+  static constexpr int FM = WM / 32, FN = WN / 32;
+  static constexpr int CPR = BK / 8;          // 16-byte chunks per A row
+  static constexpr int CPB = BN / 8;          // 16-byte chunks per B k-row
+  static constexpr int AJ = BM * CPR / NT;    // A loads per lane per stage
+  static constexpr int BJ = BK * CPB / NT;    // B loads per lane per stage
+  static constexpr int L = AJ + BJ;           // loads per lane per stage
+  static constexpr int NSTORE = FM * FN * 4;  // float4 C stores per lane (store_vec)
   static_assert(AJ * NT == BM * CPR && BJ * NT == BK * CPB && FN >= 1, "whole instructions per stage");
-  __shared__ __align__(16) uint16_t sA0[BM * BK], sA1[BM * BK], sA2[BM * BK], sA3[NST > 3 ? BM * BK : 8];
-  __shared__ __align__(16) uint16_t sB0[BK * BN], sB1[BK * BN], sB2[BK * BN], sB3[NST > 3 ? BK * BN : 8];
-  auto bufA = [&](auto I) -> uint16_t * {
-    if constexpr (decltype(I)::value == 0) return sA0;
-    else if constexpr (decltype(I)::value == 1) return sA1;
-    else if constexpr (decltype(I)::value == 2) return sA2;
-    else return sA3;
-  };
-  auto bufB = [&](auto I) -> uint16_t * {
-    if constexpr (decltype(I)::value == 0) return sB0;
-    else if constexpr (decltype(I)::value == 1) return sB1;
-    else if constexpr (decltype(I)::value == 2) return sB2;
-    else return sB3;
-  };
+  static_assert(2 * L + NSTORE <= 63, "stage loads and C stores fit the vector-memory counter");
+  // store_vec stages 4 KiB per wave in the third stage buffer: waves 0-3 in its A part
+  // (BM * 64 bytes), waves 4-7 in its B part (BN * 64 bytes)
+  static_assert(BM * 64 >= 4 * 4096 && (NW <= 4 || (NW <= 8 && BN * 64 >= (NW - 4) * 4096)), "C staging");
 
-  const int M = op.M, N = op.N, K = op.K;
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int wm = wave / WGN, wn = wave % WGN;
-  const __amdgpu_buffer_rsrc_t rA = make_rsrc(op.A, ((int64_t)(M - 1) * op.lda + K) * 2);
-  const __amdgpu_buffer_rsrc_t rB = make_rsrc(op.B, ((int64_t)(K - 1) * op.ldb + N) * 2);
-
-  // lane's chunks: instruction j of wave w fills LDS chunks (NW j + w) * 64 + lane
+  int lane, wave, wm, wn, half, col;
+  int K;
+  int64_t ldb;
+  __amdgpu_buffer_rsrc_t rA, rB;
   uint32_t va[AJ], vb[BJ];
   int ka[AJ], kb[BJ];
-#pragma unroll
-  for (int j = 0; j < AJ; ++j) {
-    const int p = (j * NW + wave) * 64 + lane, row = p / CPR, slot = p % CPR;
-    const int kc = slot ^ ((row >> 2) & 3);  // sw<32>: the logical k chunk stored in this slot
-    const int gm = m0 + row;
-    va[j] = gm < M ? (uint32_t)(((int64_t)gm * op.lda + kc * 8) * 2) : kOOB;
-    ka[j] = kc * 8;
+  f16v acc[FM][FN];
+
+  __device__ __forceinline__ void init_lanes() {
+    const int tid = threadIdx.x;
+    lane = tid & 63;
+    wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    wm = wave / WGN;
+    wn = wave % WGN;
+    half = lane >> 5;
+    col = lane & 31;
   }
+
+  // lane's chunks: instruction j of wave w fills LDS chunks (NW j + w) * 64 + lane
+  __device__ __forceinline__ void setup(const op_view &op, int m0, int n0) {
+    const int M = op.M, N = op.N;
+    K = op.K;
+    ldb = op.ldb;
+    rA = make_rsrc(op.A, ((int64_t)(M - 1) * op.lda + K) * 2);
+    rB = make_rsrc(op.B, ((int64_t)(K - 1) * op.ldb + N) * 2);
 #pragma unroll
-  for (int j = 0; j < BJ; ++j) {
-    const int p = (j * NW + wave) * 64 + lane, krow = p / CPB, slot = p % CPB;
-    const int nc = slot ^ (((krow & 3) << 2) & (CPB - 1));  // swn<BN>
-    const int gn = n0 + nc * 8;
-    vb[j] = gn < N ? (uint32_t)(((int64_t)krow * op.ldb + gn) * 2) : kOOB;
-    kb[j] = krow;
+    for (int j = 0; j < AJ; ++j) {
+      const int p = (j * NW + wave) * 64 + lane, row = p / CPR, slot = p % CPR;
+      const int kc = slot ^ ((row >> 2) & 3);  // sw<32>: the logical k chunk stored in this slot
+      const int gm = m0 + row;
+      va[j] = gm < M ? (uint32_t)(((int64_t)gm * op.lda + kc * 8) * 2) : kOOB;
+      ka[j] = kc * 8;
+    }
+#pragma unroll
+    for (int j = 0; j < BJ; ++j) {
+      const int p = (j * NW + wave) * 64 + lane, krow = p / CPB, slot = p % CPB;
+      const int nc = slot ^ (((krow & 3) << 2) & (CPB - 1));  // swn<BN>
+      const int gn = n0 + nc * 8;
+      vb[j] = gn < N ? (uint32_t)(((int64_t)krow * op.ldb + gn) * 2) : kOOB;
+      kb[j] = krow;
+    }
   }
-  // Stage at k0 into buffer I.  Lanes past M, N or K get the out-of-range offset (the
+
+  // Stage at k0 into (sa, sb).  Lanes past M, N or K get the out-of-range offset (the
   // descriptor's range check writes zeros and reads nothing); a stage at or past K reads
   // nothing and has a zero scalar offset, so no offset can wrap back into range.
-  auto issue = [&](auto I, int k0) {
-    uint16_t *sa = bufA(I), *sb = bufB(I);
+  __device__ __forceinline__ void issue(uint16_t *sa, uint16_t *sb, int k0) {
     const int lim = K - k0;
     const uint32_t soa = k0 < K ? (uint32_t)k0 * 2u : 0u;
-    const uint32_t sob = k0 < K ? (uint32_t)((int64_t)k0 * op.ldb * 2) : 0u;
+    const uint32_t sob = k0 < K ? (uint32_t)((int64_t)k0 * ldb * 2) : 0u;
 #pragma unroll
     for (int j = 0; j < AJ; ++j)
       __builtin_amdgcn_raw_ptr_buffer_load_lds(rA, (lds_void_t *)(sa + (j * NW + wave) * 64 * 8), 16,
@@ -473,21 +482,20 @@ __device__ __forceinline__ void tile_dma(const op_view &op, int m0, int n0) {
     for (int j = 0; j < BJ; ++j)
       __builtin_amdgcn_raw_ptr_buffer_load_lds(rB, (lds_void_t *)(sb + (j * NW + wave) * 64 * 8), 16,
                                                kb[j] < lim ? vb[j] : kOOB, sob, 0, 0);
-  };
+  }
 
-  f16v acc[FM][FN];
+  __device__ __forceinline__ void clear() {
 #pragma unroll
-  for (int i = 0; i < FM; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < FN; ++j)
+      for (int j = 0; j < FN; ++j)
 #pragma unroll
-      for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
+        for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
+  }
 
-  const int half = lane >> 5, col = lane & 31;
-  // no interleave strategy hint here: with two waves per SIMD the scheduler's own order ran
-  // 1-4 % faster than strategy 0
-  auto compute = [&](auto I) {
-    const uint16_t *as = bufA(I), *bs = bufB(I);
+  // (no interleave strategy hint: with two waves per SIMD the scheduler's own order ran
+  // 1-4 % faster than strategy 0)
+  __device__ __forceinline__ void compute(const uint16_t *as, const uint16_t *bs) {
 #pragma unroll
     for (int kk = 0; kk < BK / 16; ++kk) {
       const int k = kk * 16 + half * 8;
@@ -511,57 +519,130 @@ __device__ __forceinline__ void tile_dma(const op_view &op, int m0, int n0) {
         for (int j = 0; j < FN; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
     }
-  };
-  // The barrier carries no memory fence (a workgroup fence would wait for every load in
-  // flight, the later stages' too), so scheduling barriers keep every LDS read and MFMA of a
-  // stage on its side of the wait and the barrier.
-  auto step = [&](auto W) {
-    __builtin_amdgcn_sched_barrier(0);
-    wait_vmcnt<decltype(W)::value>();
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_sched_barrier(0);
-  };
-  using in_flight = std::integral_constant<int, (NST - 2) * L>;  // newest NST - 2 stages
-  using none = std::integral_constant<int, 0>;
-
-  const int nk = (K + BK - 1) / BK;
-  __syncthreads();  // LDS may still be read by the previous tile of a persistent loop
-  static_for<0, NST - 1>([&](auto I) { issue(I, decltype(I)::value * BK); });
-  step(in_flight{});
-  // whole groups of NST stages, buffers 0 .. NST-1 in turn; then the last stages
-  int kt = 0;
-  for (; kt + NST - 1 < nk; kt += NST) {
-    static_for<0, NST>([&](auto R) {
-      constexpr int r = decltype(R)::value;
-      issue(std::integral_constant<int, (r + NST - 1) % NST>{}, (kt + r + NST - 1) * BK);
-      compute(R);
-      step(in_flight{});
-    });
   }
-  // stage kt + r sits in buffer r; after the first, each waits for every load in flight
-  static_for<0, NST - 1>([&](auto R) {
-    constexpr int r = decltype(R)::value;
-    if (kt + r < nk) {
-      if constexpr (r > 0) step(none{});
-      compute(R);
-    }
-  });
-  wait_vmcnt<0>();  // stages issued past K have landed before the next tile reuses LDS
 
-  const __amdgpu_buffer_rsrc_t rC = make_rsrc(op.C, ((int64_t)(M - 1) * op.ldc + N) * 4);
-  const float alpha = op.alpha;
+  // The barrier carries no memory fence (a workgroup fence would wait for every load in
+  // flight, the later stages' too), so compiler and scheduling barriers keep every LDS read
+  // and MFMA of a stage on its side of the wait and the barrier.
+  template <int W>
+  static __device__ __forceinline__ void step() {
+    __builtin_amdgcn_sched_barrier(0);
+    asm volatile("" ::: "memory");
+    wait_vmcnt<W>();
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+  }
+
+  // C = alpha * acc, one 4-byte store per value (any ldc, any alignment)
+  __device__ __forceinline__ void store_scalar(const op_view &op, int m0, int n0) const {
+    const __amdgpu_buffer_rsrc_t rC = make_rsrc(op.C, ((int64_t)(op.M - 1) * op.ldc + op.N) * 4);
 #pragma unroll
-  for (int i = 0; i < FM; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      const int gn = n0 + wn * WN + j * 32 + col;
+      for (int j = 0; j < FN; ++j) {
+        const int gn = n0 + wn * WN + j * 32 + col;
 #pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int gm = m0 + wm * WM + i * 32 + (v >> 2) * 8 + half * 4 + (v & 3);
-        const uint32_t off = (gm < M && gn < N) ? (uint32_t)(((int64_t)gm * op.ldc + gn) * 4) : kOOB;
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, alpha * acc[i][j][v]), rC, off, 0u, 0);
+        for (int v = 0; v < 16; ++v) {
+          const int gm = m0 + wm * WM + i * 32 + (v >> 2) * 8 + half * 4 + (v & 3);
+          const uint32_t off = (gm < op.M && gn < op.N) ? (uint32_t)(((int64_t)gm * op.ldc + gn) * 4) : kOOB;
+          __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, op.alpha * acc[i][j][v]), rC, off, 0u,
+                                                kCStorePolicy);
+        }
+      }
+  }
+
+  // C = alpha * acc through a wave-private 4 KiB of LDS (stage): each 32 x 32 fragment is
+  // written as the MFMA holds it (lane l: column l % 32, rows 8 (v / 4) + 4 (l / 32) + v % 4),
+  // read back as rows of four, and stored 16 bytes per lane: NSTORE stores per lane instead
+  // of 16 per fragment.  Needs ldc % 4 == 0, a 16-byte aligned C, and N % 4 == 0 (a row of
+  // four is then wholly inside or wholly outside the block).  LDS traffic of one wave is
+  // processed in order, so no barrier is needed between the writes and the reads.
+  __device__ __forceinline__ void store_vec(const op_view &op, int m0, int n0, float *stage) const {
+    const __amdgpu_buffer_rsrc_t rC = make_rsrc(op.C, ((int64_t)(op.M - 1) * op.ldc + op.N) * 4);
+    const int rr = lane >> 3, q4 = (lane & 7) * 4;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) stage[((v >> 2) * 8 + half * 4 + (v & 3)) * 32 + col] = op.alpha * acc[i][j][v];
+        const int gn = n0 + wn * WN + j * 32 + q4;
+#pragma unroll
+        for (int r8 = 0; r8 < 4; ++r8) {
+          const int r = r8 * 8 + rr;
+          const uint4 val = *reinterpret_cast<const uint4 *>(&stage[r * 32 + q4]);
+          const int gm = m0 + wm * WM + i * 32 + r;
+          const uint32_t off = (gm < op.M && gn < op.N) ? (uint32_t)(((int64_t)gm * op.ldc + gn) * 4) : kOOB;
+          __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32, val), rC, off, 0u, kCStorePolicy);
+        }
+      }
+  }
+};
+
+// The K loop of one tile.  Stages 0 and 1 are issued already (buffers 0 and 1), and the
+// caller's wait and barrier have made stage 0 visible to every wave.  Ends with every load of
+// the tile landed, including stages issued past K, so the caller may reuse all three buffers
+// after a barrier.
+template <typename E>
+__device__ __forceinline__ void dma_kloop(E &e, uint16_t *sA0, uint16_t *sB0, uint16_t *sA1, uint16_t *sB1,
+                                          uint16_t *sA2, uint16_t *sB2) {
+  constexpr int BK = E::BK, L = E::L;
+  const int nk = (e.K + BK - 1) / BK;
+  if (nk > 0) {
+    // Stage s sits in buffer s % 3.  Stage 0 is peeled off the loop: its reads have one path
+    // into them, from the caller's wait (which may leave a previous tile's C stores in flight),
+    // and the wait after it (stage 1 landed) is where those stores drain, behind stage 0's
+    // MFMAs.  Inside a loop the compiler's own wait before a buffer's first read would merge
+    // the entry with the back edge.
+    e.issue(sA2, sB2, 2 * BK);
+    e.compute(sA0, sB0);
+    E::template step<L>();
+    // whole triples of stages from stage 1: buffers 1, 2, 0 in turn; then the last one or two
+    int kt = 1;
+    for (; kt + 2 < nk; kt += 3) {
+      e.issue(sA0, sB0, (kt + 2) * BK);
+      e.compute(sA1, sB1);
+      E::template step<L>();
+      e.issue(sA1, sB1, (kt + 3) * BK);
+      e.compute(sA2, sB2);
+      E::template step<L>();
+      e.issue(sA2, sB2, (kt + 4) * BK);
+      e.compute(sA0, sB0);
+      E::template step<L>();
+    }
+    if (kt < nk) {  // stage kt is in buffer 1, stage kt + 1 (if any) in buffer 2
+      e.compute(sA1, sB1);
+      if (kt + 1 < nk) {
+        E::template step<0>();  // stage kt + 1's loads are the newest in flight: wait for all of them
+        e.compute(sA2, sB2);
       }
     }
+  }
+  wait_vmcnt<0>();  // stages issued past K have landed before the buffers are reused
+}
+
+// store_vec's conditions
+__device__ __forceinline__ bool c_vec_ok(const op_view &op) {
+  return op.ldc % 4 == 0 && op.N % 4 == 0 && (reinterpret_cast<uintptr_t>(op.C) & 15u) == 0;
+}
+
+// One tile on its own (the single-GEMM kernel).
+template <int BM, int BN, int NT>
+__device__ __forceinline__ void tile_dma(const op_view &op, int m0, int n0) {
+  using E = dma_tile<BM, BN, NT>;
+  __shared__ __align__(16) uint16_t sA0[BM * 32], sA1[BM * 32], sA2[BM * 32];
+  __shared__ __align__(16) uint16_t sB0[32 * BN], sB1[32 * BN], sB2[32 * BN];
+  E e;
+  e.init_lanes();
+  e.setup(op, m0, n0);
+  e.clear();
+  lds_barrier();  // no wave still reads LDS (a persistent caller's previous tile)
+  e.issue(sA0, sB0, 0);
+  e.issue(sA1, sB1, E::BK);
+  E::template step<E::L>();  // stage 0 has landed in every wave
+  dma_kloop(e, sA0, sB0, sA1, sB1, sA2, sB2);
+  e.store_scalar(op, m0, n0);
 }
 
 // BF16 tiles that take the LDS-direct stage loads

@@ -93,32 +93,130 @@ __global__ __launch_bounds__((tile_threads<BM, BN, InT, S, T, FAST>::value)) __a
   tile<BM, BN, InT, S, T, FAST>(o, tm * BM, tn * BN, smem);
 }
 
+// The persistent loop of the LDS-direct tiles.  Tile boundaries overlap: when a tile's K
+// loop ends, the next ticket (drawn when the tile started) is published, the next tile's
+// first two stages are issued into the free buffers 0 and 1, and only then the tile's C goes
+// out (through the third buffer, 16 bytes per store), so the next tile waits for its first
+// stage with those stores still in flight, and they drain during its first stage's MFMAs.
+template <int BM, int BN, int NT>
+__device__ __forceinline__ void grouped_dma(const gop *__restrict__ ops, const int2 *__restrict__ items, int base,
+                                            int len, unsigned *counter, int wgs) {
+#if defined(__HIP_DEVICE_COMPILE__)  // device code only; the host pass never runs it
+  using E = dma_tile<BM, BN, NT>;
+  __shared__ __align__(16) uint16_t sA0[BM * 32], sA1[BM * 32], sA2[BM * 32];
+  __shared__ __align__(16) uint16_t sB0[32 * BN], sB1[32 * BN], sB2[32 * BN];
+  __shared__ int s_ticket[2];
+  E e;
+  e.init_lanes();
+  // C staging for the float4 stores, 4 KiB per wave: buffer 2 of A while it covers every wave
+  // (stages 0 and 1 of the next tile are in flight in buffers 0 and 1 meanwhile), else a
+  // buffer of its own.  One LDS object per pointer, so that the compiler tells these writes
+  // apart from the stage loads in flight and puts no wait for them first.
+  constexpr int NW = NT / 64;
+  constexpr bool OWN_STAGE = NW * 1024 > BM * 32 / 2;
+  __shared__ __align__(16) float sC[OWN_STAGE ? NW * 1024 : 4];
+  float *stage = OWN_STAGE ? sC + e.wave * 1024 : reinterpret_cast<float *>(sA2) + e.wave * 1024;
+  if (threadIdx.x == 0) s_ticket[0] = (int)atomicAdd(counter, 1u);
+  __syncthreads();
+  int t = __builtin_amdgcn_readfirstlane(s_ticket[0]);
+  if (t >= len) {
+    if (threadIdx.x == 0 && t == len + wgs - 1) atomicExch(counter, 0u);
+    return;
+  }
+  auto locate = [&](int ticket, op_view &v, int &m0, int &n0) {
+    const int2 it = items[base + ticket];
+    const gop o = ops[it.x];
+    int tm, tn;
+    grouped_tile(it.y, o.tiles_m, o.tiles_n, &tm, &tn);
+    v = o.v;
+    m0 = tm * BM;
+    n0 = tn * BN;
+  };
+  op_view op;
+  int m0, n0;
+  locate(t, op, m0, n0);
+  e.setup(op, m0, n0);
+  e.issue(sA0, sB0, 0);
+  e.issue(sA1, sB1, E::BK);
+  wait_vmcnt<E::L>();  // the first tile's stage 0 has landed in this wave
+  for (int par = 0;; par ^= 1) {
+    int next;  // lane 0 only; no initial value, so the atomic's result needs no early wait
+    if (threadIdx.x == 0) next = (int)atomicAdd(counter, 1u);
+    e.clear();
+    // Stage 0 has landed in every wave.  After the first tile, the previous tile's C stores
+    // (NSTORE float4 stores or four times as many scalar ones) were issued after stage 1, so
+    // a wait down to L + NSTORE covers stage 0 and leaves them in flight.
+    E::template step<E::L + E::NSTORE>();
+    dma_kloop(e, sA0, sB0, sA1, sB1, sA2, sB2);
+    if (threadIdx.x == 0) s_ticket[par ^ 1] = next;
+    lds_barrier();  // every wave's reads of this tile's stages are done; the ticket is visible
+    const int t2 = __builtin_amdgcn_readfirstlane(s_ticket[par ^ 1]);
+    const bool more = t2 < len;
+    op_view op2 = op;
+    int m2 = 0, n2 = 0;
+    if (more) {
+      locate(t2, op2, m2, n2);
+      e.setup(op2, m2, n2);
+      e.issue(sA0, sB0, 0);
+      e.issue(sA1, sB1, E::BK);
+    }
+    // uniform in fact; said so, so that the two store paths stay a plain two-way branch (a
+    // flag the compiler takes as divergent gets a third path through neither, and the next
+    // tile's first read would then wait for the stage-1 loads with no stores behind them)
+    const bool cvec = __builtin_amdgcn_readfirstlane((int)c_vec_ok(op)) != 0;
+    if (cvec) e.store_vec(op, m0, n0, stage);
+    else e.store_scalar(op, m0, n0);
+    if (!more) {
+      if (threadIdx.x == 0 && t2 == len + wgs - 1) atomicExch(counter, 0u);
+      return;
+    }
+    op = op2;
+    m0 = m2;
+    n0 = n2;
+  }
+#endif
+}
+
 // Queue x of nq holds items [qstart[x], qstart[x+1]); workgroup w serves queue w % nq
 // (the grid is a multiple of nq), with its own ticket counter at counters[64 x].
 template <int BM, int BN, typename InT, int S, int T, bool FAST>
 __global__ __launch_bounds__((tile_threads<BM, BN, InT, S, T, FAST>::value)) __attribute__((amdgpu_waves_per_eu(kernel_waves<BM, BN, InT, S, T, FAST>::value))) void gemm_grouped_kernel(const gop *__restrict__ ops, const int2 *__restrict__ items,
                                                            const int *__restrict__ qstart, int nq,
                                                            unsigned *counters) {
-  extern __shared__ __align__(16) uint16_t smem[];
-  __shared__ int s_ticket;
   const int x = (int)blockIdx.x % nq;
   const int wgs = (int)gridDim.x / nq;
   const int base = qstart[x], len = qstart[x + 1] - base;
   unsigned *counter = counters + 64 * x;
-  for (;;) {
-    if (threadIdx.x == 0) s_ticket = (int)atomicAdd(counter, 1u);
-    __syncthreads();
-    const int t = s_ticket;
-    __syncthreads();
+  if constexpr (FAST && uses_dma<BM, BN, InT, S, T>::value) {
+    grouped_dma<BM, BN, tile_threads<BM, BN, InT, S, T, FAST>::value>(ops, items, base, len, counter, wgs);
+    return;
+  }
+  extern __shared__ __align__(16) uint16_t smem[];
+  __shared__ int s_ticket[2];
+  if (threadIdx.x == 0) s_ticket[0] = (int)atomicAdd(counter, 1u);
+  __syncthreads();
+  // The next ticket is drawn when a tile starts and published when it ends, so the atomic's
+  // round trip hides behind the tile; the barriers between tiles wait for LDS only, so the
+  // tile's C stores drain while the next tile's first stage loads are in flight.
+  for (int par = 0;; par ^= 1) {
+    // uniform, so the item and op reads below are scalar loads, outside the vector-memory
+    // count that the previous tile's C stores are still in
+    const int t = __builtin_amdgcn_readfirstlane(s_ticket[par]);
     if (t >= len) {
       if (threadIdx.x == 0 && t == len + wgs - 1) atomicExch(counter, 0u);
       return;
     }
+    // set and read by lane 0 only; no initial value, so the atomic's result needs no copy
+    // (and no wait for it) where the two paths meet
+    int next;
+    if (threadIdx.x == 0) next = (int)atomicAdd(counter, 1u);
     const int2 it = items[base + t];
     const gop o = ops[it.x];
     int tm, tn;
     grouped_tile(it.y, o.tiles_m, o.tiles_n, &tm, &tn);
     tile<BM, BN, InT, S, T, FAST>(o.v, tm * BM, tn * BN, smem);
+    if (threadIdx.x == 0) s_ticket[par ^ 1] = next;
+    lds_barrier();
   }
 }
 

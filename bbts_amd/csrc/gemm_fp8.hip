@@ -338,7 +338,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void g
       for (int e = 0; e < 16; ++e) {
         const int gm = m0 + wm * WM + i * 32 + (e >> 2) * 8 + half * 4 + (e & 3);
         const uint32_t off = (gm < M && gn < N) ? (uint32_t)(((int64_t)gm * v.ldc + gn) * 4) : kOOB;
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v.alpha * acc[i][j][e]), rC, off, 0u, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v.alpha * acc[i][j][e]), rC, off, 0u,
+                                              kCStorePolicy);
       }
     }
 }
