@@ -426,6 +426,7 @@ struct dma_tile {
 
   int lane, wave, wm, wn, half, col;
   int K;
+  int rows_of;  // this wave's 32-row blocks with rows inside M: FM, FM / 2 or 0 (no MFMA work)
   int64_t ldb;
   __amdgpu_buffer_rsrc_t rA, rB;
   uint32_t va[AJ], vb[BJ];
@@ -436,8 +437,11 @@ struct dma_tile {
     const int tid = threadIdx.x;
     lane = tid & 63;
     wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    // Waves w and w + NW / 2 share a SIMD.  The second half of the waves takes the column
+    // slabs in mirrored order, so a tile cut short by N leaves each SIMD some MFMA work
+    // instead of idling whole SIMDs while the others do full work.
     wm = wave / WGN;
-    wn = wave % WGN;
+    wn = wm ? WGN - 1 - wave % WGN : wave % WGN;
     half = lane >> 5;
     col = lane & 31;
   }
@@ -447,6 +451,10 @@ struct dma_tile {
     const int M = op.M, N = op.N;
     K = op.K;
     ldb = op.ldb;
+    // A wave whose slab lies past M or N skips its MFMAs; one whose rows end within the first
+    // half of its slab runs the first FM / 2 row blocks only (the rest would land past M).
+    const int rows = M - (m0 + wm * WM), cols = N - (n0 + wn * WN);
+    rows_of = __builtin_amdgcn_readfirstlane(rows <= 0 || cols <= 0 ? 0 : rows <= WM / 2 ? FM / 2 : FM);
     rA = make_rsrc(op.A, ((int64_t)(M - 1) * op.lda + K) * 2);
     rB = make_rsrc(op.B, ((int64_t)(K - 1) * op.ldb + N) * 2);
 #pragma unroll
@@ -495,13 +503,17 @@ struct dma_tile {
 
   // (no interleave strategy hint: with two waves per SIMD the scheduler's own order ran
   // 1-4 % faster than strategy 0)
+  // row blocks 0 .. NI - 1 of this wave's slab
+  template <int NI>
   __device__ __forceinline__ void compute(const uint16_t *as, const uint16_t *bs) {
+    static_assert(FM % 2 == 0, "half slabs");
+    if constexpr (NI == 0) return;
 #pragma unroll
     for (int kk = 0; kk < BK / 16; ++kk) {
       const int k = kk * 16 + half * 8;
-      bf16x8 af[FM], bfr[FN];
+      bf16x8 af[NI > 0 ? NI : 1], bfr[FN];
 #pragma unroll
-      for (int i = 0; i < FM; ++i)
+      for (int i = 0; i < NI; ++i)
         af[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&as[sw<BK>(wm * WM + i * 32 + col, k)]));
 #pragma unroll
       for (int j = 0; j < FN; ++j) {
@@ -514,7 +526,7 @@ struct dma_tile {
         bfr[j] = __builtin_bit_cast(bf16x8, both);
       }
 #pragma unroll
-      for (int i = 0; i < FM; ++i)
+      for (int i = 0; i < NI; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
@@ -584,9 +596,9 @@ struct dma_tile {
 // caller's wait and barrier have made stage 0 visible to every wave.  Ends with every load of
 // the tile landed, including stages issued past K, so the caller may reuse all three buffers
 // after a barrier.
-template <typename E>
-__device__ __forceinline__ void dma_kloop(E &e, uint16_t *sA0, uint16_t *sB0, uint16_t *sA1, uint16_t *sB1,
-                                          uint16_t *sA2, uint16_t *sB2) {
+template <int NI, typename E>
+__device__ __forceinline__ void dma_kloop_n(E &e, uint16_t *sA0, uint16_t *sB0, uint16_t *sA1, uint16_t *sB1,
+                                            uint16_t *sA2, uint16_t *sB2) {
   constexpr int BK = E::BK, L = E::L;
   const int nk = (e.K + BK - 1) / BK;
   if (nk > 0) {
@@ -596,30 +608,41 @@ __device__ __forceinline__ void dma_kloop(E &e, uint16_t *sA0, uint16_t *sB0, ui
     // MFMAs.  Inside a loop the compiler's own wait before a buffer's first read would merge
     // the entry with the back edge.
     e.issue(sA2, sB2, 2 * BK);
-    e.compute(sA0, sB0);
+    e.template compute<NI>(sA0, sB0);
     E::template step<L>();
     // whole triples of stages from stage 1: buffers 1, 2, 0 in turn; then the last one or two
     int kt = 1;
     for (; kt + 2 < nk; kt += 3) {
       e.issue(sA0, sB0, (kt + 2) * BK);
-      e.compute(sA1, sB1);
+      e.template compute<NI>(sA1, sB1);
       E::template step<L>();
       e.issue(sA1, sB1, (kt + 3) * BK);
-      e.compute(sA2, sB2);
+      e.template compute<NI>(sA2, sB2);
       E::template step<L>();
       e.issue(sA2, sB2, (kt + 4) * BK);
-      e.compute(sA0, sB0);
+      e.template compute<NI>(sA0, sB0);
       E::template step<L>();
     }
     if (kt < nk) {  // stage kt is in buffer 1, stage kt + 1 (if any) in buffer 2
-      e.compute(sA1, sB1);
+      e.template compute<NI>(sA1, sB1);
       if (kt + 1 < nk) {
         E::template step<0>();  // stage kt + 1's loads are the newest in flight: wait for all of them
-        e.compute(sA2, sB2);
+        e.template compute<NI>(sA2, sB2);
       }
     }
   }
   wait_vmcnt<0>();  // stages issued past K have landed before the buffers are reused
+}
+
+// The K loop for this wave's row blocks, chosen once per tile: every block, the first half
+// (rows end within the slab's first half) or none (the slab lies past M or N; the wave still
+// issues its share of the stage loads and meets every barrier).
+template <typename E>
+__device__ __forceinline__ void dma_kloop(E &e, uint16_t *sA0, uint16_t *sB0, uint16_t *sA1, uint16_t *sB1,
+                                          uint16_t *sA2, uint16_t *sB2) {
+  if (e.rows_of == E::FM) dma_kloop_n<E::FM>(e, sA0, sB0, sA1, sB1, sA2, sB2);
+  else if (e.rows_of == E::FM / 2) dma_kloop_n<E::FM / 2>(e, sA0, sB0, sA1, sB1, sA2, sB2);
+  else dma_kloop_n<0>(e, sA0, sB0, sA1, sB1, sA2, sB2);
 }
 
 // store_vec's conditions

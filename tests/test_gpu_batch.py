@@ -57,6 +57,50 @@ def test_grouped_bf16_policies(ctx, orc, torch_cuda, policy):
     b.close()
 
 
+# Tiles cut short by M or N: slabs of a tile wholly past M or N (their waves skip the
+# MFMAs), slabs with rows ending in their first half (half the row blocks), and N % 4 != 0
+# (one 4-byte C store per value instead of 16-byte stores).
+EDGE_SHAPES = [(65, 300, 96), (130, 200, 64), (200, 70, 40), (257, 513, 100), (33, 17, 9), (384, 448, 160),
+               (130, 201, 64), (96, 1000, 32)]
+
+
+@pytest.mark.parametrize("cfg", range(6))
+def test_single_launch_every_config_edges(ctx, orc, torch_cuda, cfg):
+    """One plain launch per op with a given tile config (the autotuner's path)."""
+    import ctypes
+    from bbts_amd import call
+    from bbts_amd.device import GemmDesc
+    torch = torch_cuda
+    print("single launch cfg %d" % cfg, flush=True)
+    ops = _stream(torch, orc, EDGE_SHAPES, dtype=1, seed=70 + cfg)
+    for (M, N, K, a, bb, ta, tb, C) in ops:
+        d = GemmDesc(M, N, K, 1, ta.data_ptr(), tb.data_ptr(), C.data_ptr(), K, N, N)
+        call("bbts_hip_gemm_desc_launch", ctx.h, ctypes.byref(d), 1, cfg, None)
+    ctx.sync()
+    _check(orc, ops, 2e-6)
+
+
+@pytest.mark.parametrize("policy", [100, 105])
+def test_grouped_edges_forced_config(ctx, orc, torch_cuda, policy):
+    """The persistent launch on one tile config over the edge shapes, twice in a row (the
+    second launch's tiles follow the first's C stores on every workgroup)."""
+    torch = torch_cuda
+    from bbts_amd.device import Batch
+    print("grouped edges policy %d" % policy, flush=True)
+    ops = _stream(torch, orc, EDGE_SHAPES * 3, dtype=1, seed=90 + policy)
+    b = Batch(ctx)
+    for (M, N, K, a, bb, ta, tb, C) in ops:
+        b.add(M, N, K, ta.data_ptr(), tb.data_ptr(), C.data_ptr(), dtype=1, precision=1)
+    b.plan(policy)
+    for _ in range(2):
+        for op in ops:
+            op[7].fill_(float("nan"))
+        b.launch()
+        ctx.sync()
+        _check(orc, ops, 2e-6)
+    b.close()
+
+
 @pytest.mark.parametrize("prec,tol", [(2, 4e-5), (3, 2e-6)])
 def test_grouped_fp32_split(ctx, orc, torch_cuda, prec, tol):
     torch = torch_cuda
