@@ -6,11 +6,13 @@
 // chunks, bin-packs the chunks onto one queue per XCD by predicted cost
 // (largest first, i.e. LPT), and a persistent grid of min(items, CUs x
 // occupancy) workgroups pulls items in queue order from per-queue ticket
-// counters.  The counter resets itself: tickets are handed out
-// until every workgroup has drawn one past the end, so the workgroup that
-// draws the very last ticket (items + grid - 1) knows no further draw can
-// happen and zeroes it.  A captured hipGraph therefore replays with no memset
-// node and no host work.
+// counters.  The counters reset themselves, so a captured hipGraph replays with
+// no memset node and no host work: in the register-staged kernel every
+// workgroup draws once past the end of its own queue, so the workgroup that
+// draws the very last ticket (items + workgroups - 1) knows no further draw
+// can happen and zeroes it; in the LDS-direct kernel a workgroup whose queue
+// runs dry takes tiles from the other queues, and the last workgroup to finish
+// (counted on one more counter) zeroes every counter.
 #include "gemm_bf16.cuh"
 
 #include <algorithm>
@@ -99,13 +101,13 @@ __global__ __launch_bounds__((tile_threads<BM, BN, InT, S, T, FAST>::value)) __a
 // out (through the third buffer, 16 bytes per store), so the next tile waits for its first
 // stage with those stores still in flight, and they drain during its first stage's MFMAs.
 template <int BM, int BN, int NT>
-__device__ __forceinline__ void grouped_dma(const gop *__restrict__ ops, const int2 *__restrict__ items, int base,
-                                            int len, unsigned *counter, int wgs) {
+__device__ __forceinline__ void grouped_dma(const gop *__restrict__ ops, const int2 *__restrict__ items,
+                                            const int *__restrict__ qstart, int nq, unsigned *counters) {
 #if defined(__HIP_DEVICE_COMPILE__)  // device code only; the host pass never runs it
   using E = dma_tile<BM, BN, NT>;
   __shared__ __align__(16) uint16_t sA0[BM * 32], sA1[BM * 32], sA2[BM * 32];
   __shared__ __align__(16) uint16_t sB0[32 * BN], sB1[32 * BN], sB2[32 * BN];
-  __shared__ int s_ticket[2];
+  __shared__ int s_ticket[2], s_steal;
   E e;
   e.init_lanes();
   // C staging for the float4 stores, 4 KiB per wave: buffer 2 of A while it covers every wave
@@ -116,11 +118,39 @@ __device__ __forceinline__ void grouped_dma(const gop *__restrict__ ops, const i
   constexpr bool OWN_STAGE = NW * 1024 > BM * 32 / 2;
   __shared__ __align__(16) float sC[OWN_STAGE ? NW * 1024 : 4];
   float *stage = OWN_STAGE ? sC + e.wave * 1024 : reinterpret_cast<float *>(sA2) + e.wave * 1024;
-  if (threadIdx.x == 0) s_ticket[0] = (int)atomicAdd(counter, 1u);
+  // The workgroup serves its own XCD's queue first; once that queue is drawn dry it takes
+  // tiles from the next queues in turn (one draw each, at most nq in all), so that no XCD
+  // idles while another still has tiles.  q, base and len are uniform.
+  int q = (int)blockIdx.x % nq, visited = 0;
+  int base = qstart[q], len = qstart[q + 1] - base;
+  unsigned *done = counters + 64 * nq;
+  // a valid ticket of queue q (moving q on as queues run dry), or -1 when all are dry
+  auto valid_ticket = [&](int t) {
+    while (t >= len) {
+      if (++visited == nq) return -1;
+      q = q + 1 == nq ? 0 : q + 1;
+      base = qstart[q];
+      len = qstart[q + 1] - base;
+      if (threadIdx.x == 0) s_steal = (int)atomicAdd(counters + 64 * q, 1u);
+      lds_barrier();
+      t = __builtin_amdgcn_readfirstlane(s_steal);
+      lds_barrier();  // every wave has read s_steal before lane 0 writes it again
+    }
+    return t;
+  };
+  // Every workgroup ends here once, after its last draw; the last one to arrive resets every
+  // queue counter and the arrival count for the next launch (no other workgroup draws again).
+  auto finish = [&]() {
+    if (threadIdx.x == 0 && atomicAdd(done, 1u) == gridDim.x - 1) {
+      for (int y = 0; y < nq; ++y) atomicExch(counters + 64 * y, 0u);
+      atomicExch(done, 0u);
+    }
+  };
+  if (threadIdx.x == 0) s_ticket[0] = (int)atomicAdd(counters + 64 * q, 1u);
   __syncthreads();
-  int t = __builtin_amdgcn_readfirstlane(s_ticket[0]);
-  if (t >= len) {
-    if (threadIdx.x == 0 && t == len + wgs - 1) atomicExch(counter, 0u);
+  int t = valid_ticket(__builtin_amdgcn_readfirstlane(s_ticket[0]));
+  if (t < 0) {
+    finish();
     return;
   }
   auto locate = [&](int ticket, op_view &v, int &m0, int &n0) {
@@ -141,7 +171,7 @@ __device__ __forceinline__ void grouped_dma(const gop *__restrict__ ops, const i
   wait_vmcnt<E::L>();  // the first tile's stage 0 has landed in this wave
   for (int par = 0;; par ^= 1) {
     int next;  // lane 0 only; no initial value, so the atomic's result needs no early wait
-    if (threadIdx.x == 0) next = (int)atomicAdd(counter, 1u);
+    if (threadIdx.x == 0) next = (int)atomicAdd(counters + 64 * q, 1u);
     e.clear();
     // Stage 0 has landed in every wave.  After the first tile, the previous tile's C stores
     // (NSTORE float4 stores or four times as many scalar ones) were issued after stage 1, so
@@ -150,8 +180,8 @@ __device__ __forceinline__ void grouped_dma(const gop *__restrict__ ops, const i
     dma_kloop(e, sA0, sB0, sA1, sB1, sA2, sB2);
     if (threadIdx.x == 0) s_ticket[par ^ 1] = next;
     lds_barrier();  // every wave's reads of this tile's stages are done; the ticket is visible
-    const int t2 = __builtin_amdgcn_readfirstlane(s_ticket[par ^ 1]);
-    const bool more = t2 < len;
+    const int t2 = valid_ticket(__builtin_amdgcn_readfirstlane(s_ticket[par ^ 1]));
+    const bool more = t2 >= 0;
     op_view op2 = op;
     int m2 = 0, n2 = 0;
     if (more) {
@@ -167,7 +197,7 @@ __device__ __forceinline__ void grouped_dma(const gop *__restrict__ ops, const i
     if (cvec) e.store_vec(op, m0, n0, stage);
     else e.store_scalar(op, m0, n0);
     if (!more) {
-      if (threadIdx.x == 0 && t2 == len + wgs - 1) atomicExch(counter, 0u);
+      finish();
       return;
     }
     op = op2;
@@ -178,7 +208,9 @@ __device__ __forceinline__ void grouped_dma(const gop *__restrict__ ops, const i
 }
 
 // Queue x of nq holds items [qstart[x], qstart[x+1]); workgroup w serves queue w % nq
-// (the grid is a multiple of nq), with its own ticket counter at counters[64 x].
+// (the grid is a multiple of nq), with its own ticket counter at counters[64 x]; the
+// LDS-direct kernel then moves on to the other queues and counts finished workgroups at
+// counters[64 nq].
 template <int BM, int BN, typename InT, int S, int T, bool FAST>
 __global__ __launch_bounds__((tile_threads<BM, BN, InT, S, T, FAST>::value)) __attribute__((amdgpu_waves_per_eu(kernel_waves<BM, BN, InT, S, T, FAST>::value))) void gemm_grouped_kernel(const gop *__restrict__ ops, const int2 *__restrict__ items,
                                                            const int *__restrict__ qstart, int nq,
@@ -188,7 +220,7 @@ __global__ __launch_bounds__((tile_threads<BM, BN, InT, S, T, FAST>::value)) __a
   const int base = qstart[x], len = qstart[x + 1] - base;
   unsigned *counter = counters + 64 * x;
   if constexpr (FAST && uses_dma<BM, BN, InT, S, T>::value) {
-    grouped_dma<BM, BN, tile_threads<BM, BN, InT, S, T, FAST>::value>(ops, items, base, len, counter, wgs);
+    grouped_dma<BM, BN, tile_threads<BM, BN, InT, S, T, FAST>::value>(ops, items, qstart, nq, counters);
     return;
   }
   extern __shared__ __align__(16) uint16_t smem[];
@@ -824,7 +856,9 @@ int bbts_hip_batch_plan(bbts_batch *b, int policy, bbts_costmodel *cm) {
     bytes += qstart.size() * sizeof(int);
     bytes = (bytes + 255) / 256 * 256;
     grp.counter_off = bytes;
-    bytes += 64 * sizeof(unsigned) * (size_t)nq;  // one 256-byte line per queue counter
+    // one 256-byte line per queue counter, and one for the LDS-direct kernel's count of
+    // finished workgroups
+    bytes += 64 * sizeof(unsigned) * (size_t)(nq + 1);
     gqstart.push_back(std::move(qstart));
     b->groups.push_back(grp);
     gops.push_back(std::move(ops));
