@@ -663,6 +663,148 @@ int parse_ewb(const int32_t *w, int nw, int rank_l, const int64_t *dl, int rank_
   return BBTS_OK;
 }
 
+// One operand ("staged") runs contiguously along an output mode b other than mode 0 and with
+// a stride along output mode 0; the other ("direct") is contiguous or broadcast along mode 0.
+// 64 x 64 tiles of (a = output mode 0, b), batched over up to two more output modes: the
+// staged operand is read along b into LDS, the direct operand and the output run along a.
+struct ewbt_t {
+  uint32_t A, B;                    // extents of a and b
+  uint32_t os_b, ds_a, ds_b, ss_a;  // output b stride; direct a (0 or 1) and b strides; staged a stride
+  fastdiv ta, tb, e0;               // tiles along a and b, first batch extent
+  uint32_t os0, os1, ds0, ds1, ss0, ss1;  // batch strides: output, direct, staged
+};
+
+// V4: A and B are multiples of 4, the buffers 16-byte aligned and every stride a multiple of
+// 4 (a broadcast direct operand aside), so global reads and writes move float4s.
+template <int OP, bool LSTAGED, bool V4>
+__global__ __launch_bounds__(kT) void ewb_transpose_kernel(const float *__restrict__ l, const float *__restrict__ r,
+                                                          float *__restrict__ out, ewbt_t t, float alpha, float beta) {
+  __shared__ float tile[64][65];
+  uint32_t id = blockIdx.x;
+  uint32_t q = fd_div(t.ta, id);
+  const uint32_t a0 = (id - q * t.ta.d) * 64;
+  id = q;
+  q = fd_div(t.tb, id);
+  const uint32_t b0 = (id - q * t.tb.d) * 64;
+  id = q;
+  q = fd_div(t.e0, id);
+  const uint32_t c0 = id - q * t.e0.d, c1 = q;
+  const float *st = (LSTAGED ? l : r) + c0 * t.ss0 + c1 * t.ss1;
+  const float *di = (LSTAGED ? r : l) + c0 * t.ds0 + c1 * t.ds1;
+  float *dst = out + c0 * t.os0 + c1 * t.os1;
+  // op(alpha * lhs, beta * rhs) with the staged value in its operand's place
+  auto val = [&](float sv, float dv) {
+    return LSTAGED ? bin_op<OP>(alpha * sv, beta * dv) : bin_op<OP>(alpha * dv, beta * sv);
+  };
+  if (V4) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int x = threadIdx.x + k * kT, bq = (x & 15) * 4, al = x >> 4;
+      const uint32_t a = a0 + al, b = b0 + bq;
+      if (a < t.A && b < t.B) {
+        const float4 v = *reinterpret_cast<const float4 *>(st + a * t.ss_a + b);
+        tile[al][bq] = v.x;
+        tile[al][bq + 1] = v.y;
+        tile[al][bq + 2] = v.z;
+        tile[al][bq + 3] = v.w;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int x = threadIdx.x + k * kT, aq = (x & 15) * 4, bl = x >> 4;
+      const uint32_t a = a0 + aq, b = b0 + bl;
+      if (a < t.A && b < t.B) {
+        float4 d;
+        if (t.ds_a) {
+          d = *reinterpret_cast<const float4 *>(di + a + b * t.ds_b);
+        } else {
+          const float v = di[b * t.ds_b];
+          d = make_float4(v, v, v, v);
+        }
+        float4 o;
+        o.x = val(tile[aq][bl], d.x);
+        o.y = val(tile[aq + 1][bl], d.y);
+        o.z = val(tile[aq + 2][bl], d.z);
+        o.w = val(tile[aq + 3][bl], d.w);
+        *reinterpret_cast<float4 *>(dst + a + b * t.os_b) = o;
+      }
+    }
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int x = threadIdx.x + k * kT, bl = x & 63, al = x >> 6;
+    const uint32_t a = a0 + al, b = b0 + bl;
+    if (a < t.A && b < t.B) tile[al][bl] = st[a * t.ss_a + b];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int x = threadIdx.x + k * kT, al = x & 63, bl = x >> 6;
+    const uint32_t a = a0 + al, b = b0 + bl;
+    if (a < t.A && b < t.B) dst[a + b * t.os_b] = val(tile[al][bl], di[a * t.ds_a + b * t.ds_b]);
+  }
+}
+
+// The tiled form for g when one operand is strided along output mode 0 and contiguous along
+// another mode b of at least 32, and the other operand is contiguous or broadcast along mode 0
+// (also at least 32); false when g is not of that form.
+template <int OP>
+bool launch_ewb_transpose(const float *l, const float *r, float *out, const ewb_t &g, float alpha, float beta,
+                          hipStream_t s) {
+  const uint32_t A = g.od[0].d;
+  if (g.rank < 2 || A < 32) return false;
+  for (int lst = 0; lst < 2; ++lst) {
+    const uint32_t *S = lst ? g.ls : g.rs, *D = lst ? g.rs : g.ls;
+    if (D[0] > 1 || S[0] <= 1) continue;
+    int b = -1;
+    for (int m = 1; m < g.rank; ++m)
+      if (S[m] == 1) b = m;
+    if (b < 0 || g.od[b].d < 32) continue;
+    uint32_t os[4], o = 1;
+    for (int m = 0; m < g.rank; ++m) {
+      os[m] = o;
+      o *= g.od[m].d;
+    }
+    ewbt_t t{};
+    t.A = A;
+    t.B = g.od[b].d;
+    t.os_b = os[b];
+    t.ds_a = D[0];
+    t.ds_b = D[b];
+    t.ss_a = S[0];
+    uint32_t ext[2] = {1, 1};
+    int nb = 0;
+    for (int m = 1; m < g.rank; ++m) {
+      if (m == b) continue;
+      ext[nb] = g.od[m].d;
+      (nb ? t.os1 : t.os0) = os[m];
+      (nb ? t.ds1 : t.ds0) = D[m];
+      (nb ? t.ss1 : t.ss0) = S[m];
+      ++nb;
+    }
+    const uint32_t ta = (A + 63) / 64, tb = (t.B + 63) / 64;
+    t.ta = make_fastdiv(ta);
+    t.tb = make_fastdiv(tb);
+    t.e0 = make_fastdiv(ext[0]);
+    const uint64_t blocks = (uint64_t)ta * tb * ext[0] * ext[1];
+    const auto al16 = [](const void *p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; };
+    const float *st = lst ? l : r, *di = lst ? r : l;
+    bool v4 = A % 4 == 0 && t.B % 4 == 0 && al16(out) && al16(st) && t.ss_a % 4 == 0 && t.ss0 % 4 == 0 &&
+              t.ss1 % 4 == 0 && (t.ds_a == 0 || (al16(di) && t.ds_b % 4 == 0 && t.ds0 % 4 == 0 && t.ds1 % 4 == 0));
+    if (lst) {
+      if (v4) hipLaunchKernelGGL((ewb_transpose_kernel<OP, true, true>), dim3((uint32_t)blocks), dim3(kT), 0, s, l, r, out, t, alpha, beta);
+      else hipLaunchKernelGGL((ewb_transpose_kernel<OP, true, false>), dim3((uint32_t)blocks), dim3(kT), 0, s, l, r, out, t, alpha, beta);
+    } else {
+      if (v4) hipLaunchKernelGGL((ewb_transpose_kernel<OP, false, true>), dim3((uint32_t)blocks), dim3(kT), 0, s, l, r, out, t, alpha, beta);
+      else hipLaunchKernelGGL((ewb_transpose_kernel<OP, false, false>), dim3((uint32_t)blocks), dim3(kT), 0, s, l, r, out, t, alpha, beta);
+    }
+    return true;
+  }
+  return false;
+}
+
 template <int OP, bool SCALE>
 int launch_ewb_op(const float *l, const float *r, float *out, const ewb_t &g, uint32_t n, float alpha, float beta,
                   hipStream_t s) {
@@ -672,6 +814,8 @@ int launch_ewb_op(const float *l, const float *r, float *out, const ewb_t &g, ui
                     reinterpret_cast<uintptr_t>(r) % 16 == 0 && reinterpret_cast<uintptr_t>(out) % 16 == 0;
     hipLaunchKernelGGL((ewb_flat_kernel<OP, SCALE>), dim3(blocks_for(vec ? n / 4 : n, kT * 4)), dim3(kT), 0, s, l, ls,
                        r, rs, out, n, alpha, beta, vec);
+  } else if (SCALE && launch_ewb_transpose<OP>(l, r, out, g, alpha, beta, s)) {
+    // tiled (one operand transposed against the output)
   } else {
     const auto al16 = [](const void *p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; };
     const uint32_t d0 = g.od[0].d;

@@ -290,7 +290,7 @@ def suite(torch, ctx, steps):
          lambda h, i: ctx.cu_ew_raw(ew_params(1, 1.0, (2, 3, 0, 1)), (64, 64, 32, 32), X[i], Y[i], h)),
         ("cutensor ewb add, 2048x2048 + bias over mode 1", 8.0 * mn + 4.0 * n,
          lambda h, i: ctx.cu_ewb_raw(ewb_params(0, 1.0, 1.0, (0, 1), (1,)), sq, X[i], (n,), bias, Y[i], h)),
-        ("cutensor ewb mul, 2048x2048 * transposed 2048x2048 (gather)", 12.0 * mn,
+        ("cutensor ewb mul, 2048x2048 * transposed 2048x2048 (LDS-tiled transpose)", 12.0 * mn,
          lambda h, i: ctx.cu_ewb_raw(ewb_params(3, 1.0, 1.0, (0, 1), (1, 0)), sq, X[i], sq, X[i + 4], Y[i], h)),
         ("cutensor ewb_castable add 2048x2048", 12.0 * mn,
          lambda h, i: ctx.cu_ewb_castable_raw(0, X[i], X[i + 4], Y[i], h)),

@@ -118,6 +118,16 @@ EWB_GPU_CASES = EWB_CASES + [
     ((512, 768), (1, 0), (512, 768), (1, 0)),
     ((96, 80, 12), (0, 1, 2), (12, 96), (2, 0)),
     ((1, 300, 1), (0, 1, 2), (300,), (1,)),
+    # one operand transposed against the output (64 x 64 LDS tiles): float4 and scalar forms,
+    # the transposed operand on either side, one and two batch modes, and a direct operand
+    # that is a vector along mode 0 or broadcast along it
+    ((300, 200), (0, 1), (200, 300), (1, 0)),
+    ((130, 97), (0, 1), (97, 130), (1, 0)),
+    ((256, 128), (1, 0), (128, 256), (0, 1)),
+    ((64, 48, 5), (0, 1, 2), (48, 64, 5), (1, 0, 2)),
+    ((40, 3, 36, 2), (0, 1, 2, 3), (36, 3, 40, 2), (2, 1, 0, 3)),
+    ((256,), (0,), (96, 256), (1, 0)),
+    ((96,), (1,), (96, 256), (1, 0)),
 ]
 
 
