@@ -1062,7 +1062,9 @@ int run_reduction(bbts_hip_ctx *ctx, const int32_t *w, int nw, int rank, const i
   if (segments > 0) {
     S = (uint64_t)segments;
   } else if (wave) {
-    if (num_out < 8192 && A > 512) S = std::min<uint64_t>((8192 + num_out - 1) / num_out, (A + 511) / 512);
+    // split only below 2048 waves (8 per CU): at 2048 x 2048 -> 2048 one pass ran 5.7 us and
+    // the 4-way split with its combine 7.5 us
+    if (num_out < 2048 && A > 512) S = std::min<uint64_t>((2048 + num_out - 1) / num_out, (A + 511) / 512);
   } else {
     if (num_out < 262144 && A > 32) S = std::min<uint64_t>((262144 + num_out - 1) / num_out, (A + 15) / 16);
   }
