@@ -31,11 +31,18 @@ struct gop {
   int tiles_m;
 };
 
+// One work item of the LDS-direct grouped kernel with everything its tile needs, so that
+// locating the next tile is one load rather than an item load and then a dependent op load
+struct tile_item {
+  op_view v;
+  int m0, n0;
+};
+
 // Tile t of an op in groups of 8 tile rows, column by column inside a group: the tiles that
 // run at the same time on one XCD (its 32 CUs x 2 workgroups) cover an 8 x 8 block of the
 // output and share 8 A row panels and 8 B column panels in that XCD's L2, where a row-major
 // order would stream every B column panel once per tile row.
-__device__ inline void grouped_tile(int t, int tiles_m, int tiles_n, int *tm, int *tn) {
+__host__ __device__ inline void grouped_tile(int t, int tiles_m, int tiles_n, int *tm, int *tn) {
   constexpr int G = 8;
   const int width = G * tiles_n;
   const int first_m = (t / width) * G;
@@ -101,8 +108,8 @@ __global__ __launch_bounds__((tile_threads<BM, BN, InT, S, T, FAST>::value)) __a
 // out (through the third buffer, 16 bytes per store), so the next tile waits for its first
 // stage with those stores still in flight, and they drain during its first stage's MFMAs.
 template <int BM, int BN, int NT>
-__device__ __forceinline__ void grouped_dma(const gop *__restrict__ ops, const int2 *__restrict__ items,
-                                            const int *__restrict__ qstart, int nq, unsigned *counters) {
+__device__ __forceinline__ void grouped_dma(const tile_item *__restrict__ titems, const int *__restrict__ qstart,
+                                            int nq, unsigned *counters) {
 #if defined(__HIP_DEVICE_COMPILE__)  // device code only; the host pass never runs it
   using E = dma_tile<BM, BN, NT>;
   __shared__ __align__(16) uint16_t sA0[BM * 32], sA1[BM * 32], sA2[BM * 32];
@@ -154,13 +161,10 @@ __device__ __forceinline__ void grouped_dma(const gop *__restrict__ ops, const i
     return;
   }
   auto locate = [&](int ticket, op_view &v, int &m0, int &n0) {
-    const int2 it = items[base + ticket];
-    const gop o = ops[it.x];
-    int tm, tn;
-    grouped_tile(it.y, o.tiles_m, o.tiles_n, &tm, &tn);
-    v = o.v;
-    m0 = tm * BM;
-    n0 = tn * BN;
+    const tile_item ti = titems[base + ticket];
+    v = ti.v;
+    m0 = ti.m0;
+    n0 = ti.n0;
   };
   op_view op;
   int m0, n0;
@@ -214,13 +218,14 @@ __device__ __forceinline__ void grouped_dma(const gop *__restrict__ ops, const i
 template <int BM, int BN, typename InT, int S, int T, bool FAST>
 __global__ __launch_bounds__((tile_threads<BM, BN, InT, S, T, FAST>::value)) __attribute__((amdgpu_waves_per_eu(kernel_waves<BM, BN, InT, S, T, FAST>::value))) void gemm_grouped_kernel(const gop *__restrict__ ops, const int2 *__restrict__ items,
                                                            const int *__restrict__ qstart, int nq,
-                                                           unsigned *counters) {
+                                                           unsigned *counters,
+                                                           const tile_item *__restrict__ titems) {
   const int x = (int)blockIdx.x % nq;
   const int wgs = (int)gridDim.x / nq;
   const int base = qstart[x], len = qstart[x + 1] - base;
   unsigned *counter = counters + 64 * x;
   if constexpr (FAST && uses_dma<BM, BN, InT, S, T>::value) {
-    grouped_dma<BM, BN, tile_threads<BM, BN, InT, S, T, FAST>::value>(ops, items, qstart, nq, counters);
+    grouped_dma<BM, BN, tile_threads<BM, BN, InT, S, T, FAST>::value>(titems, qstart, nq, counters);
     return;
   }
   extern __shared__ __align__(16) uint16_t smem[];
@@ -258,10 +263,11 @@ struct cfg_shape {
 };
 constexpr cfg_shape kCfgs[] = {{256, 128}, {128, 128}, {128, 64}, {64, 64}, {64, 128}, {256, 256}};
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
+inline int pbm_of(int cfg) { return kCfgs[cfg].bm; }
 
 // variant: 0 = BF16 operands (S=T=1); 1,2,3 = FP32 operands split into 1,2,3 slices (T=S)
 typedef void (*single_fn)(op_view, int64_t, int64_t, int64_t, int, int, uint32_t *, uint32_t, uint32_t);
-typedef void (*grouped_fn)(const gop *, const int2 *, const int *, int, unsigned *);
+typedef void (*grouped_fn)(const gop *, const int2 *, const int *, int, unsigned *, const tile_item *);
 
 template <int BM, int BN, bool FAST>
 struct cfg_kernels {
@@ -488,7 +494,7 @@ struct bbts_batch {
     bool fast = true;
     int nitems = 0, grid = 0, nq = 1;
     double flops = 0.0;
-    size_t ops_off = 0, items_off = 0, qstart_off = 0, counter_off = 0;
+    size_t ops_off = 0, items_off = 0, qstart_off = 0, counter_off = 0, titems_off = 0, ntitems = 0;
   };
   std::vector<group> groups;
   void *dev = nullptr;
@@ -766,6 +772,7 @@ int bbts_hip_batch_plan(bbts_batch *b, int policy, bbts_costmodel *cm) {
   size_t bytes = 0;
   std::vector<std::vector<gop>> gops;
   std::vector<std::vector<int2>> gitems;
+  std::vector<std::vector<tile_item>> gtitems;
   std::vector<std::vector<int>> gqstart;
   for (auto &kv : gmap) {
     auto &idx = kv.second;
@@ -859,10 +866,26 @@ int bbts_hip_batch_plan(bbts_batch *b, int policy, bbts_costmodel *cm) {
     // one 256-byte line per queue counter, and one for the LDS-direct kernel's count of
     // finished workgroups
     bytes += 64 * sizeof(unsigned) * (size_t)(nq + 1);
+    // the LDS-direct kernel's self-contained items (its op view and tile origin each)
+    std::vector<tile_item> titems;
+    if (grp.fast && grp.variant == 0 && pbm_of(grp.cfg) == 256) {
+      titems.reserve(items.size());
+      for (const int2 &it : items) {
+        const gop &o = ops[(size_t)it.x];
+        int tm, tn;
+        grouped_tile(it.y, o.tiles_m, o.tiles_n, &tm, &tn);
+        titems.push_back(tile_item{o.v, tm * kCfgs[grp.cfg].bm, tn * kCfgs[grp.cfg].bn});
+      }
+    }
+    bytes = (bytes + 255) / 256 * 256;
+    grp.titems_off = bytes;
+    grp.ntitems = titems.size();
+    bytes += titems.size() * sizeof(tile_item);
     gqstart.push_back(std::move(qstart));
     b->groups.push_back(grp);
     gops.push_back(std::move(ops));
     gitems.push_back(std::move(items));
+    gtitems.push_back(std::move(titems));
   }
   if (bytes > b->dev_bytes) {
     if (b->dev) {
@@ -880,6 +903,8 @@ int bbts_hip_batch_plan(bbts_batch *b, int policy, bbts_costmodel *cm) {
       std::memcpy(host.data() + grp.ops_off, gops[gi].data(), gops[gi].size() * sizeof(gop));
       std::memcpy(host.data() + grp.items_off, gitems[gi].data(), gitems[gi].size() * sizeof(int2));
       std::memcpy(host.data() + grp.qstart_off, gqstart[gi].data(), gqstart[gi].size() * sizeof(int));
+      if (!gtitems[gi].empty())
+        std::memcpy(host.data() + grp.titems_off, gtitems[gi].data(), gtitems[gi].size() * sizeof(tile_item));
     }
     BBTS_HIP_CHECK(hipMemcpy(b->dev, host.data(), bytes, hipMemcpyHostToDevice));
   }
@@ -897,6 +922,9 @@ int bbts_hip_batch_launch_group(bbts_batch *b, int g, bbts_stream_t stream) {
   if (rc) return rc;
   const auto &grp = b->groups[(size_t)g];
   char *base = static_cast<char *>(b->dev);
+  // the LDS-direct kernel reads its items from the self-contained table only
+  BBTS_REQUIRE(!(grp.fast && grp.variant == 0 && pbm_of(grp.cfg) == 256) || grp.ntitems == (size_t)grp.nitems,
+               BBTS_E_INVALID, "grouped launch without its tile table");
   grouped_fn fn = get_grouped(grp.cfg, grp.variant, grp.fast);
   int bytes = dyn_smem_bytes(grp.cfg, grp.variant, grp.fast);
   if ((rc = ensure_smem((const void *)fn, bytes))) return rc;
@@ -905,7 +933,8 @@ int bbts_hip_batch_launch_group(bbts_batch *b, int g, bbts_stream_t stream) {
                      reinterpret_cast<const gop *>(base + grp.ops_off),
                      reinterpret_cast<const int2 *>(base + grp.items_off),
                      reinterpret_cast<const int *>(base + grp.qstart_off), grp.nq,
-                     reinterpret_cast<unsigned *>(base + grp.counter_off));
+                     reinterpret_cast<unsigned *>(base + grp.counter_off),
+                     grp.ntitems ? reinterpret_cast<const tile_item *>(base + grp.titems_off) : nullptr);
   BBTS_LAUNCH_CHECK();
   return BBTS_OK;
 }
