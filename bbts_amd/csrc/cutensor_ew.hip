@@ -30,6 +30,14 @@ namespace bbts_hip {
 namespace {
 
 constexpr int kT = 256;
+// Non-temporal 16-byte store for outputs written once and not read again by the kernel:
+// measured on 2048 x 2048 blocks, ew relu +4 %, the LDS-tiled transposes +19-35 %,
+// ewb_castable +8 %; the broadcast-operand ewb form ran 10 % slower with it and keeps
+// ordinary stores.
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void store_nt(float4 *p, const float4 &v) {
+  __builtin_nontemporal_store(*reinterpret_cast<const nt_f4 *>(&v), reinterpret_cast<nt_f4 *>(p));
+}
 
 // n / d for 0 <= n < 2^31, 1 <= d < 2^31: q = (umulhi(n, m) + n) >> s with s = ceil(log2 d)
 // and m = floor(2^32 (2^s - d) / d) + 1 (Granlund and Montgomery, "Division by invariant
@@ -107,7 +115,7 @@ __global__ __launch_bounds__(kT) void ew_flat_kernel(const float *__restrict__ i
     v.y = ew_fn<F>(v.y, c) * alpha;
     v.z = ew_fn<F>(v.z, c) * alpha;
     v.w = ew_fn<F>(v.w, c) * alpha;
-    out4[i] = v;
+    store_nt(&out4[i], v);
   }
   for (uint32_t i = n4 * 4 + blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT)
     out[i] = ew_fn<F>(in[i], c) * alpha;
@@ -192,7 +200,7 @@ __global__ __launch_bounds__(kT) void ew_transpose_kernel(const float *__restric
         o.y = ew_fn<F>(tile[aq + 1][bl], c) * alpha;
         o.z = ew_fn<F>(tile[aq + 2][bl], c) * alpha;
         o.w = ew_fn<F>(tile[aq + 3][bl], c) * alpha;
-        *reinterpret_cast<float4 *>(dst + a + b * t.os_b) = o;
+        store_nt(reinterpret_cast<float4 *>(dst + a + b * t.os_b), o);
       }
     }
     return;
@@ -234,7 +242,7 @@ __global__ __launch_bounds__(kT) void ewb_flat_kernel(const float *__restrict__ 
       o.y = ewb_val<OP, SCALE>(a.y, b.y, alpha, beta);
       o.z = ewb_val<OP, SCALE>(a.z, b.z, alpha, beta);
       o.w = ewb_val<OP, SCALE>(a.w, b.w, alpha, beta);
-      o4[i] = o;
+      store_nt(&o4[i], o);
     }
     return;
   }
@@ -727,7 +735,7 @@ __global__ __launch_bounds__(kT) void ewb_transpose_kernel(const float *__restri
         o.y = val(tile[aq + 1][bl], d.y);
         o.z = val(tile[aq + 2][bl], d.z);
         o.w = val(tile[aq + 3][bl], d.w);
-        *reinterpret_cast<float4 *>(dst + a + b * t.os_b) = o;
+        store_nt(reinterpret_cast<float4 *>(dst + a + b * t.os_b), o);
       }
     }
     return;
