@@ -2,7 +2,8 @@
 // grouped persistent stream kernel.
 //
 // One workgroup = 256 lanes = 4 wave64s in a 2x2 grid computes a BM x BN tile
-// of C = alpha * A·B (row-major A: M x K, B: K x N, C: M x N, FP32 out).
+// of C = alpha * A·B (row-major A: M x K, B: K x N, C: M x N, FP32 out); the
+// LDS-direct BF16 256 x 256 tile (dma_tile below) runs 512 lanes in 2 x 4.
 // Per K stage (BK = 64 for one BF16 slice, 32 for split operands) both
 // operands are staged into LDS as k-contiguous rows of BK BF16 values, with
 // the 16-byte chunks of a row XOR-swizzled by the row index, so that eight
@@ -420,9 +421,9 @@ struct dma_tile {
   static constexpr int NSTORE = FM * FN * 4;  // float4 C stores per lane (store_vec)
   static_assert(AJ * NT == BM * CPR && BJ * NT == BK * CPB && FN >= 1, "whole instructions per stage");
   static_assert(2 * L + NSTORE <= 63, "stage loads and C stores fit the vector-memory counter");
-  // store_vec stages 4 KiB per wave in the third stage buffer: waves 0-3 in its A part
-  // (BM * 64 bytes), waves 4-7 in its B part (BN * 64 bytes)
-  static_assert(BM * 64 >= 4 * 4096 && (NW <= 4 || (NW <= 8 && BN * 64 >= (NW - 4) * 4096)), "C staging");
+  // store_vec stages 4 KiB per wave: the persistent caller puts it in the A part of the third
+  // stage buffer (BM * 64 bytes) when that covers every wave, else in a buffer of its own
+  static_assert(NW <= 8, "C staging");
 
   int lane, wave, wm, wn, half, col;
   int K;
