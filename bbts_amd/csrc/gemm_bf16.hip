@@ -840,11 +840,16 @@ int bbts_hip_batch_plan(bbts_batch *b, int policy, bbts_costmodel *cm) {
     // whose output writes and operand loads outweigh their MFMA work, to a bandwidth-bound
     // tail; the packing order keeps the two kinds mixed (measured 3-5 % faster on the
     // 512-op stream).
+    // The longest tiles (K >= kLongK) go first in their queue, the rest keep the packing
+    // order: no workgroup starts a long tile near the end of the launch.
+    constexpr int kLongK = 2048;
     for (int x = 0; x < nq; ++x) {
       auto &qq = queue[(size_t)x];
       qstart[(size_t)x] = (int)items.size();
-      for (const chunk *c : qq)
-        for (int t = c->t0; t < c->t1; ++t) items.push_back(make_int2(c->op, t));
+      for (int pass = 0; pass < 2; ++pass)
+        for (const chunk *c : qq)
+          if ((ops[(size_t)c->op].v.K >= kLongK) == (pass == 0))
+            for (int t = c->t0; t < c->t1; ++t) items.push_back(make_int2(c->op, t));
     }
     qstart[(size_t)nq] = (int)items.size();
     grp.nitems = (int)items.size();
