@@ -151,7 +151,9 @@ int bbts_hip_cu_ewb_castable_raw(bbts_hip_ctx *ctx, int op, const float *lhs, co
  * out = alpha * fold(op) over the input modes absent from ordering, out mode r = input mode
  * ordering[r].  When the input's contiguous mode is kept and no split is taken, the fold is
  * the reference's left fold in column-major order, bit for bit.  segments: 0 picks the split
- * for occupancy; k > 0 forces k contiguous segments folded separately, then folded in order. */
+ * for occupancy; k > 0 forces k contiguous segments folded separately, then combined: in
+ * segment order for up to 16 segments, by a wave per output (lanes over the segments, then
+ * a butterfly) above that. */
 int bbts_hip_cu_reduction(bbts_hip_ctx *ctx, const bbts_tensor_t *in, bbts_tensor_t *out, const int32_t *params,
                           int nparams, bbts_stream_t stream);
 int bbts_hip_cu_reduction_raw(bbts_hip_ctx *ctx, const int32_t *params, int nparams, int rank, const int64_t *dims,
