@@ -840,9 +840,12 @@ int bbts_hip_batch_plan(bbts_batch *b, int policy, bbts_costmodel *cm) {
     // whose output writes and operand loads outweigh their MFMA work, to a bandwidth-bound
     // tail; the packing order keeps the two kinds mixed (measured 3-5 % faster on the
     // 512-op stream).
-    // The longest tiles (K >= kLongK) go first in their queue, the rest keep the packing
-    // order: no workgroup starts a long tile near the end of the launch.
-    constexpr int kLongK = 2048;
+    // Tiles with K >= kLongK go first in their queue, the rest keep the packing order, so the
+    // launch ends on a mix of short tiles (a K loop within a few times the ~6 us a tile
+    // spends outside it) rather than on a long tile started late.  On the 512-op stream:
+    // threshold 2048 +0.4 % over the plain packing order, 1024 +0.5 % more, 512 and 384
+    // +1.1 % more, 256 back to the 2048 level (the tail is then store-bound tiles only).
+    constexpr int kLongK = 512;
     for (int x = 0; x < nq; ++x) {
       auto &qq = queue[(size_t)x];
       qstart[(size_t)x] = (int)items.size();
