@@ -840,18 +840,20 @@ int bbts_hip_batch_plan(bbts_batch *b, int policy, bbts_costmodel *cm) {
     // whose output writes and operand loads outweigh their MFMA work, to a bandwidth-bound
     // tail; the packing order keeps the two kinds mixed (measured 3-5 % faster on the
     // 512-op stream).
-    // Tiles with K >= kLongK go first in their queue, the rest keep the packing order, so the
+    // Tiles with K >= long_k go first in their queue, the rest keep the packing order, so the
     // launch ends on a mix of short tiles (a K loop within a few times the ~6 us a tile
     // spends outside it) rather than on a long tile started late.  On the 512-op stream:
     // threshold 2048 +0.4 % over the plain packing order, 1024 +0.5 % more, 512 and 384
     // +1.1 % more, 256 back to the 2048 level (the tail is then store-bound tiles only).
-    constexpr int kLongK = 512;
+    // (Measured on the LDS-direct launch, which also takes tiles across queues; the other
+    // configs keep the packing order.)
+    const int long_k = grp.fast && grp.variant == 0 && pbm_of(grp.cfg) == 256 ? 512 : 0;
     for (int x = 0; x < nq; ++x) {
       auto &qq = queue[(size_t)x];
       qstart[(size_t)x] = (int)items.size();
       for (int pass = 0; pass < 2; ++pass)
         for (const chunk *c : qq)
-          if ((ops[(size_t)c->op].v.K >= kLongK) == (pass == 0))
+          if ((ops[(size_t)c->op].v.K >= long_k) == (pass == 0))
             for (int t = c->t0; t < c->t1; ++t) items.push_back(make_int2(c->op, t));
     }
     qstart[(size_t)nq] = (int)items.size();
