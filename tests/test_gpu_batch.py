@@ -143,6 +143,49 @@ def test_graph_replays_match_eager(ctx, orc, torch_cuda):
     b.close()
 
 
+@pytest.mark.parametrize("policy", [100, 105])
+def test_grouped_queues_taken_across_xcds(ctx, orc, torch_cuda, policy):
+    """Enough uneven ops for one queue per XCD, on the LDS-direct tiles (256 x 128, 256 x 256):
+    workgroups whose queue runs dry take tiles from the others, and the last one to finish
+    resets every counter.  Two eager launches in a row (the second relies on that reset),
+    then graph replays with eager launches between them, each bit-identical to the first
+    launch, which is checked against the FP64 product."""
+    torch = torch_cuda
+    from bbts_amd.device import Batch
+    rng = np.random.default_rng(5 + policy)
+    shapes = [tuple(int(x) for x in 64 * rng.integers(1, 13, size=3)) for _ in range(40)]
+    shapes += [(768, 768, 2048), (1024, 512, 64)]  # one long tile run and one short one
+    ops = _stream(torch, orc, shapes, dtype=1, seed=300 + policy)
+    b = Batch(ctx)
+    for (M, N, K, a, bb, ta, tb, C) in ops:
+        b.add(M, N, K, ta.data_ptr(), tb.data_ptr(), C.data_ptr(), dtype=1, precision=1)
+    b.plan(policy)
+    from bbts_amd import lib
+    print("grouped queues policy %d, groups %d" % (policy, lib().bbts_hip_batch_num_groups(b.h)), flush=True)
+    b.launch()
+    ctx.sync()
+    _check(orc, ops, 2e-6)
+    first = [C.clone() for (*_, C) in ops]
+    for (*_, C) in ops:
+        C.fill_(float("nan"))
+    b.launch()
+    ctx.sync()
+    for f, (*_, C) in zip(first, ops):
+        assert torch.equal(f, C)
+    b.capture()
+    for rep in range(3):
+        for (*_, C) in ops:
+            C.fill_(float("nan"))
+        torch.cuda.synchronize()
+        b.replay()
+        ctx.sync()
+        for f, (*_, C) in zip(first, ops):
+            assert torch.equal(f, C), rep
+        b.launch()
+        ctx.sync()
+    b.close()
+
+
 def test_tuner_fills_table(ctx, orc, torch_cuda):
     torch = torch_cuda
     from bbts_amd.device import Batch
